@@ -1,0 +1,355 @@
+// CKKS-level kernels for gfx950: rescale, Galois automorphisms, key-switch
+// multiply-accumulate, ModDown and plaintext addition.  Each reproduces the
+// reference CPU loop body (csrc/ops/ckks/cpu/ckks_cpu.cpp) value for value.
+#include "fh_common.h"
+#include "fh_views.h"
+
+namespace fh {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPerThread = 4;
+constexpr int kChunk = kThreads * kPerThread;
+
+// rescale_drop_leading_prime_{nearest,truncate} (ckks_cpu.cpp:204-275):
+// c'_i = canonicalize(REDC((c_i - c_drop) * q_drop^{-1} R) [+1]).
+template <bool kNearest>
+__global__ __launch_bounds__(kThreads) void k_rescale(View3 out, View3 remaining,
+                                                      View3 dropped, Vec inverse,
+                                                      ParamView params, int64_t half_drop,
+                                                      int64_t n) {
+  const int64_t l = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const Mont m = load_mont(params, l);
+  const int64_t inv = inverse[l];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kChunk + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int64_t i = base + e * kThreads;
+    if (i >= n) break;
+    const int64_t d = dropped.at(b, 0, i);
+    int64_t v = mont_mul_split(remaining.at(b, l, i) - d, inv, m);
+    if constexpr (kNearest) v += d > half_drop ? 1 : 0;
+    out.at(b, l, i) = canonicalize(v, m.twice_q);
+  }
+}
+
+// apply_coefficient_galois_automorphism (ckks_cpu.cpp:323-405).
+__global__ __launch_bounds__(kThreads) void k_galois_coefficient(
+    View3 out, View3 in, const int32_t* src, int64_t src_stride, const int8_t* sign,
+    int64_t sign_stride, Vec twice, int64_t n) {
+  const int64_t l = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const int64_t t2 = twice[l];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kChunk + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int64_t i = base + e * kThreads;
+    if (i >= n) break;
+    int64_t v = in.at(b, l, src[i * src_stride]);
+    if (sign[i * sign_stride] == -1) v = -v;
+    out.at(b, l, i) = canonicalize(shift_positive(v, t2), t2);
+  }
+}
+
+// apply_ntt_galois_automorphism (ckks_cpu.cpp:407-472): a pure gather.
+__global__ __launch_bounds__(kThreads) void k_galois_ntt(View3 out, View3 in,
+                                                         const int32_t* src,
+                                                         int64_t src_stride, int64_t n) {
+  const int64_t l = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kChunk + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int64_t i = base + e * kThreads;
+    if (i >= n) break;
+    out.at(b, l, i) = in.at(b, l, src[i * src_stride]);
+  }
+}
+
+// keyswitch_accumulate_digit_products_ (ckks_cpu.cpp:603-708):
+// acc_k += REDC(digit * key_k[key_row_start + limb]) lazily, k = 0, 1.
+__global__ __launch_bounds__(kThreads) void k_keyswitch_mac(View3 acc0, View3 acc1,
+                                                            View3 digit, View3 key,
+                                                            ParamView params,
+                                                            int64_t key_row_start,
+                                                            int64_t n) {
+  const int64_t l = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const Mont m = load_mont(params, l);
+  const int64_t krow = key_row_start + l;
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kChunk + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int64_t i = base + e * kThreads;
+    if (i >= n) break;
+    const int64_t x = digit.at(b, l, i);
+    const int64_t p0 = mont_mul(x, key.at(0, krow, i), m);
+    const int64_t p1 = mont_mul(x, key.at(1, krow, i), m);
+    acc0.at(b, l, i) = add_lazy(acc0.at(b, l, i), p0, m.twice_q);
+    acc1.at(b, l, i) = add_lazy(acc1.at(b, l, i), p1, m.twice_q);
+  }
+}
+
+// keyswitch_moddown_qp_to_q (ckks_cpu.cpp:474-601): floor(x / P) mod q_i by
+// sequentially dropping canonical P residues, one coefficient per thread.
+constexpr int kMaxP = 8;
+
+__global__ __launch_bounds__(kThreads) void k_moddown(View3 out, View3 qv, View3 pv,
+                                                      const int64_t* inverse,
+                                                      int64_t inv_drop_stride,
+                                                      int64_t inv_limb_stride,
+                                                      ParamView params, int q_count,
+                                                      int p_count, int64_t n) {
+  const int64_t b = blockIdx.y;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  int64_t chain[kMaxP];
+#pragma unroll
+  for (int r = 0; r < kMaxP; ++r) chain[r] = r < p_count ? pv.at(b, r, i) : 0;
+  // P-chain: each P residue is made exact modulo the lower P primes.
+  for (int row = p_count - 2; row >= 0; --row) {
+    int64_t value = chain[row];
+    const Mont m = load_mont(params, q_count + row);
+    for (int lower = p_count - 1; lower > row; --lower) {
+      const int64_t diff = sub_lazy(value, chain[lower], m.twice_q);
+      value = mont_mul_split(
+          diff,
+          inverse[(p_count - lower - 1) * inv_drop_stride + (row + q_count) * inv_limb_stride],
+          m);
+    }
+    chain[row] = value;
+  }
+  for (int row = 0; row < q_count; ++row) {
+    const Mont m = load_mont(params, row);
+    int64_t value = mont_mul(qv.at(b, row, i), m.r2, m);
+    for (int p = p_count - 1; p >= 0; --p) {
+      const int64_t pm = mont_mul(chain[p], m.r2, m);
+      value = sub_lazy(value, pm, m.twice_q);
+      value = mont_mul_split(
+          value, inverse[(p_count - p - 1) * inv_drop_stride + row * inv_limb_stride], m);
+    }
+    value = mont_mul(value, 1, m);
+    out.at(b, row, i) = canonicalize(value, m.twice_q);
+  }
+}
+
+// Plaintext addition (ckks_cpu.cpp:54-146): c + p with p in Montgomery form,
+// result canonical standard.  Layouts: dense, cyclic / contiguous compressed
+// and strided with an implicit per-limb fill value.
+enum PlainLayout : int { kDense = 0, kCyclic = 1, kContiguous = 2, kStrided = 3 };
+
+template <int LAYOUT>
+__global__ __launch_bounds__(kThreads) void k_add_plain(View3 out, View3 ct, View3 pt,
+                                                        const int64_t* implicit,
+                                                        int64_t pt_batch_count,
+                                                        int64_t limb_count,
+                                                        ParamView params, int64_t support,
+                                                        int64_t n) {
+  const int64_t l = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const Mont m = load_mont(params, l);
+  const int64_t repeat = n / support;
+  const int64_t pb = pt_batch_count == 1 ? 0 : b;
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kChunk + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int64_t i = base + e * kThreads;
+    if (i >= n) break;
+    int64_t p;
+    if constexpr (LAYOUT == kDense) {
+      p = pt.at(b, l, i);
+    } else if constexpr (LAYOUT == kCyclic) {
+      p = pt.at(b, l, i % support);
+    } else if constexpr (LAYOUT == kContiguous) {
+      p = pt.at(b, l, i / repeat);
+    } else {
+      p = implicit[pb * limb_count + l];
+      if (i % repeat == 0) p = pt.at(b, l, i / repeat);
+    }
+    int64_t v = mont_mul(ct.at(b, l, i), m.r2, m);
+    v = add_lazy(v, p, m.twice_q);
+    v = mont_mul(v, 1, m);
+    out.at(b, l, i) = canonicalize(v, m.twice_q);
+  }
+}
+
+// CkksEngine.multiply (ckks_engine.py:1583-1598) in one pass: d0 = a0 b0,
+// d1 = add_lazy(a0 b1, a1 b0), d2 = a1 b1 with the reference's REDC.  Reads
+// four limb sets and writes three instead of the 12 + 3 of five separate ops.
+__global__ __launch_bounds__(kThreads) void k_tensor_product(View3 d0, View3 d1, View3 d2, View3 a0,
+                                                             View3 a1, View3 b0, View3 b1,
+                                                             ParamView params, int64_t n) {
+  const int64_t l = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const Mont m = load_mont(params, l);
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kChunk + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int64_t i = base + e * kThreads;
+    if (i >= n) break;
+    const int64_t x0 = a0.at(b, l, i), x1 = a1.at(b, l, i);
+    const int64_t y0 = b0.at(b, l, i), y1 = b1.at(b, l, i);
+    d0.at(b, l, i) = mont_mul(x0, y0, m);
+    d1.at(b, l, i) = add_lazy(mont_mul(x0, y1, m), mont_mul(x1, y0, m), m.twice_q);
+    d2.at(b, l, i) = mont_mul(x1, y1, m);
+  }
+}
+
+int check_grid(int64_t batch, int64_t limbs, int64_t n) {
+  FH_CHECK_ARG(batch >= 0 && limbs >= 0 && n >= 0, "negative extent");
+  FH_CHECK_ARG(batch <= 65535 && limbs <= 65535,
+               "batch (%lld) or limb (%lld) extent exceeds the launch grid",
+               (long long)batch, (long long)limbs);
+  return kOk;
+}
+
+}  // namespace
+}  // namespace fh
+
+using namespace fh;
+
+FH_API int fh_ckks_rescale(fh_view3 out, fh_view3 remaining, fh_view3 dropped,
+                           fh_vec drop_prime_inverse_mont, fh_table2 rns_params,
+                           int64_t batch, int64_t limbs, int64_t n,
+                           int64_t half_drop_prime, int32_t nearest, void* stream) {
+  int st = check_grid(batch, limbs, n);
+  if (st != kOk) return st;
+  if (batch == 0 || limbs == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(out.data && remaining.data && dropped.data && drop_prime_inverse_mont.data &&
+                   rns_params.data,
+               "rescale: null operand");
+  ParamView pv{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  dim3 grid(ceil_div(n, kChunk), limbs, batch);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (nearest) {
+    hipLaunchKernelGGL(k_rescale<true>, grid, dim3(kThreads), 0, s, View3(out), View3(remaining),
+                       View3(dropped), Vec(drop_prime_inverse_mont), pv, half_drop_prime, n);
+  } else {
+    hipLaunchKernelGGL(k_rescale<false>, grid, dim3(kThreads), 0, s, View3(out), View3(remaining),
+                       View3(dropped), Vec(drop_prime_inverse_mont), pv, half_drop_prime, n);
+  }
+  FH_CHECK_LAUNCH("ckks rescale");
+  return kOk;
+}
+
+FH_API int fh_ckks_galois_coefficient(fh_view3 out, fh_view3 in, const int32_t* source_indices,
+                                      int64_t index_stride, const int8_t* source_sign,
+                                      int64_t sign_stride, fh_vec twice_modulus,
+                                      int64_t batch, int64_t limbs, int64_t n,
+                                      void* stream) {
+  int st = check_grid(batch, limbs, n);
+  if (st != kOk) return st;
+  if (batch == 0 || limbs == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(out.data && in.data && source_indices && source_sign && twice_modulus.data,
+               "coefficient galois: null operand");
+  dim3 grid(ceil_div(n, kChunk), limbs, batch);
+  hipLaunchKernelGGL(k_galois_coefficient, grid, dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), View3(out), View3(in), source_indices,
+                     index_stride, source_sign, sign_stride, Vec(twice_modulus), n);
+  FH_CHECK_LAUNCH("ckks coefficient galois");
+  return kOk;
+}
+
+FH_API int fh_ckks_galois_ntt(fh_view3 out, fh_view3 in, const int32_t* source_indices,
+                              int64_t index_stride, int64_t batch, int64_t limbs, int64_t n,
+                              void* stream) {
+  int st = check_grid(batch, limbs, n);
+  if (st != kOk) return st;
+  if (batch == 0 || limbs == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(out.data && in.data && source_indices, "ntt galois: null operand");
+  dim3 grid(ceil_div(n, kChunk), limbs, batch);
+  hipLaunchKernelGGL(k_galois_ntt, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream),
+                     View3(out), View3(in), source_indices, index_stride, n);
+  FH_CHECK_LAUNCH("ckks ntt galois");
+  return kOk;
+}
+
+FH_API int fh_ckks_keyswitch_accumulate(fh_view3 acc0, fh_view3 acc1, fh_view3 digit,
+                                        fh_view3 key_digit, fh_table2 rns_params,
+                                        int64_t key_row_start, int64_t batch,
+                                        int64_t limbs, int64_t n, void* stream) {
+  int st = check_grid(batch, limbs, n);
+  if (st != kOk) return st;
+  if (batch == 0 || limbs == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(acc0.data && acc1.data && digit.data && key_digit.data && rns_params.data,
+               "keyswitch accumulate: null operand");
+  FH_CHECK_ARG(key_row_start >= 0, "negative key row start");
+  ParamView pv{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  dim3 grid(ceil_div(n, kChunk), limbs, batch);
+  hipLaunchKernelGGL(k_keyswitch_mac, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream),
+                     View3(acc0), View3(acc1), View3(digit), View3(key_digit), pv,
+                     key_row_start, n);
+  FH_CHECK_LAUNCH("ckks keyswitch accumulate");
+  return kOk;
+}
+
+FH_API int fh_ckks_moddown(fh_view3 out, fh_view3 q_residues, fh_view3 p_residues,
+                           fh_table2 p_drop_inverses, fh_table2 rns_params, int64_t batch,
+                           int64_t q_count, int64_t p_count, int64_t n, void* stream) {
+  int st = check_grid(batch, q_count, n);
+  if (st != kOk) return st;
+  FH_CHECK_ARG(p_count >= 1 && p_count <= kMaxP, "ModDown supports 1..8 P rows");
+  if (batch == 0 || q_count == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(out.data && q_residues.data && p_residues.data && p_drop_inverses.data &&
+                   rns_params.data,
+               "moddown: null operand");
+  ParamView pv{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  dim3 grid(ceil_div(n, kThreads), batch);
+  hipLaunchKernelGGL(k_moddown, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream),
+                     View3(out), View3(q_residues), View3(p_residues), p_drop_inverses.data,
+                     p_drop_inverses.stride_row, p_drop_inverses.stride_col, pv,
+                     static_cast<int>(q_count), static_cast<int>(p_count), n);
+  FH_CHECK_LAUNCH("ckks moddown");
+  return kOk;
+}
+
+FH_API int fh_ckks_add_plaintext(int32_t layout, fh_view3 out, fh_view3 ciphertext,
+                                 fh_view3 plaintext, const int64_t* implicit_plaintext,
+                                 int64_t plaintext_batch_count, fh_table2 rns_params,
+                                 int64_t batch, int64_t limbs, int64_t n, int64_t support,
+                                 void* stream) {
+  int st = check_grid(batch, limbs, n);
+  if (st != kOk) return st;
+  if (batch == 0 || limbs == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(out.data && ciphertext.data && plaintext.data && rns_params.data,
+               "plaintext add: null operand");
+  if (layout == kDense) {
+    support = n;
+  } else {
+    FH_CHECK_ARG(support > 0 && support < n && (support & (support - 1)) == 0 &&
+                     n % support == 0,
+                 "compressed support must be a power-of-two divisor of N");
+  }
+  FH_CHECK_ARG(layout != kStrided || implicit_plaintext, "strided plaintext needs fill values");
+  ParamView pv{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  dim3 grid(ceil_div(n, kChunk), limbs, batch);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  View3 o(out), c(ciphertext), p(plaintext);
+  switch (layout) {
+    case kDense: hipLaunchKernelGGL(k_add_plain<kDense>, grid, dim3(kThreads), 0, s, o, c, p, implicit_plaintext, plaintext_batch_count, limbs, pv, support, n); break;
+    case kCyclic: hipLaunchKernelGGL(k_add_plain<kCyclic>, grid, dim3(kThreads), 0, s, o, c, p, implicit_plaintext, plaintext_batch_count, limbs, pv, support, n); break;
+    case kContiguous: hipLaunchKernelGGL(k_add_plain<kContiguous>, grid, dim3(kThreads), 0, s, o, c, p, implicit_plaintext, plaintext_batch_count, limbs, pv, support, n); break;
+    case kStrided: hipLaunchKernelGGL(k_add_plain<kStrided>, grid, dim3(kThreads), 0, s, o, c, p, implicit_plaintext, plaintext_batch_count, limbs, pv, support, n); break;
+    default: set_error("unknown plaintext layout %d", layout); return kInvalidArgument;
+  }
+  FH_CHECK_LAUNCH("ckks plaintext add");
+  return kOk;
+}
+
+FH_API int fh_ckks_tensor_product(fh_view3 d0, fh_view3 d1, fh_view3 d2, fh_view3 a0, fh_view3 a1,
+                                  fh_view3 b0, fh_view3 b1, fh_table2 rns_params, int64_t batch,
+                                  int64_t limbs, int64_t n, void* stream) {
+  int st = check_grid(batch, limbs, n);
+  if (st != kOk) return st;
+  if (batch == 0 || limbs == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(d0.data && d1.data && d2.data && a0.data && a1.data && b0.data && b1.data &&
+                   rns_params.data,
+               "tensor product: null operand");
+  ParamView pv{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  dim3 grid(ceil_div(n, kChunk), limbs, batch);
+  hipLaunchKernelGGL(k_tensor_product, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream),
+                     View3(d0), View3(d1), View3(d2), View3(a0), View3(a1), View3(b0), View3(b1), pv, n);
+  FH_CHECK_LAUNCH("ckks tensor product");
+  return kOk;
+}

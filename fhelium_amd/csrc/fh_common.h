@@ -1,0 +1,227 @@
+// Shared device arithmetic, status handling and launch helpers for the gfx950
+// CKKS backend.  Everything here is integer-exact: every routine returns the
+// same residue (same representative, not merely the same class mod q) as the
+// reference CPU arithmetic in csrc/ops/common/cpu/montgomery.h:125-215.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#define FH_API extern "C" __attribute__((visibility("default")))
+
+namespace fh {
+
+// ---------------------------------------------------------------------------
+// Status reporting for the C ABI.  Entry points return 0 on success and a
+// negative code on failure; the message is kept per host thread.
+// ---------------------------------------------------------------------------
+enum Status : int {
+  kOk = 0,
+  kInvalidArgument = -1,
+  kLaunchFailed = -2,
+  kUnsupported = -3,
+};
+
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define FH_CHECK_ARG(cond, ...)              \
+  do {                                       \
+    if (!(cond)) {                           \
+      ::fh::set_error(__VA_ARGS__);          \
+      return ::fh::kInvalidArgument;         \
+    }                                        \
+  } while (0)
+
+#define FH_CHECK_LAUNCH(what)                                              \
+  do {                                                                     \
+    hipError_t fh_err_ = hipGetLastError();                                \
+    if (fh_err_ != hipSuccess) {                                           \
+      ::fh::set_error("%s: kernel launch failed: %s", what,                \
+                      hipGetErrorString(fh_err_));                         \
+      return ::fh::kLaunchFailed;                                          \
+    }                                                                      \
+  } while (0)
+
+// Reference parameter tensor rows, csrc/ops/common/rns_parameters.h:6-16.
+enum RnsParam : int {
+  kTwiceModulus = 0,
+  kModulusLo = 1,
+  kModulusHi = 2,
+  kNegInvLo = 3,
+  kNegInvHi = 4,
+  kR2 = 5,
+  kScaledR2 = 6,
+  kNInvMontgomery = 7,
+};
+
+constexpr int kHalfRadixBits = 31;   // R = 2^62 split into 31-bit halves
+constexpr int kRadixBits = 62;
+constexpr uint64_t kRadixMask = (uint64_t{1} << kRadixBits) - 1;
+
+// Strided view of the [8, limb] parameter table.
+struct ParamView {
+  const int64_t* base;
+  int64_t row_stride;   // stride between parameter kinds
+  int64_t limb_stride;  // stride between limbs
+
+  __device__ __forceinline__ int64_t at(int kind, int64_t limb) const {
+    return base[kind * row_stride + limb * limb_stride];
+  }
+};
+
+// Montgomery constants for one prime, reconstructed from the split-word
+// parameter rows exactly as the reference loads them.
+struct Mont {
+  uint64_t q;
+  uint64_t k;       // -q^{-1} mod 2^62
+  int64_t twice_q;
+  int64_t r2;       // R^2 mod q
+  int64_t ninv;     // N^{-1} R mod q
+};
+
+__device__ __forceinline__ Mont load_mont(const ParamView& p, int64_t limb) {
+  Mont m;
+  m.twice_q = p.at(kTwiceModulus, limb);
+  m.q = static_cast<uint64_t>(p.at(kModulusLo, limb)) +
+        (static_cast<uint64_t>(p.at(kModulusHi, limb)) << kHalfRadixBits);
+  m.k = static_cast<uint64_t>(p.at(kNegInvLo, limb)) +
+        (static_cast<uint64_t>(p.at(kNegInvHi, limb)) << kHalfRadixBits);
+  m.r2 = p.at(kR2, limb);
+  m.ninv = p.at(kNInvMontgomery, limb);
+  return m;
+}
+
+// REDC with R = 2^62 followed by one conditional subtraction, i.e. the
+// reference's `multiply` (montgomery.h:125-143) bit for bit, including the
+// 128-bit wrap-around of its unsigned __int128 intermediate.
+__device__ __forceinline__ int64_t mont_mul(int64_t lhs, int64_t rhs,
+                                            uint64_t q, uint64_t k) {
+  const uint64_t a = static_cast<uint64_t>(lhs);
+  const uint64_t b = static_cast<uint64_t>(rhs);
+  const uint64_t lo = a * b;
+  const uint64_t hi = __umul64hi(a, b);
+  const uint64_t m = (lo * k) & kRadixMask;
+  const uint64_t mq_lo = m * q;
+  const uint64_t mq_hi = __umul64hi(m, q);
+  const uint64_t s_lo = lo + mq_lo;
+  const uint64_t s_hi = hi + mq_hi + (s_lo < lo ? 1u : 0u);
+  const uint64_t r = (s_lo >> kRadixBits) | (s_hi << (64 - kRadixBits));
+  return static_cast<int64_t>(r < q ? r : r - q);
+}
+
+__device__ __forceinline__ int64_t mont_mul(int64_t a, int64_t b,
+                                            const Mont& m) {
+  return mont_mul(a, b, m.q, m.k);
+}
+
+// montgomery.h:148-172: signed operands are first reduced with C `%`.
+__device__ __forceinline__ int64_t canonical_operand(int64_t v, uint64_t q) {
+  int64_t c = v % static_cast<int64_t>(q);
+  if (c < 0) c += static_cast<int64_t>(q);
+  return c;
+}
+
+__device__ __forceinline__ int64_t mont_mul_split(int64_t a, int64_t b,
+                                                  const Mont& m) {
+  return mont_mul(canonical_operand(a, m.q), canonical_operand(b, m.q), m);
+}
+
+__device__ __forceinline__ int64_t add_lazy(int64_t a, int64_t b,
+                                            int64_t twice_q) {
+  const int64_t s = a + b;
+  return s < twice_q ? s : s - twice_q;
+}
+
+__device__ __forceinline__ int64_t sub_lazy(int64_t a, int64_t b,
+                                            int64_t twice_q) {
+  const int64_t d = a + twice_q - b;
+  return d < twice_q ? d : d - twice_q;
+}
+
+__device__ __forceinline__ int64_t canonicalize(int64_t v, int64_t twice_q) {
+  const int64_t q = twice_q >> 1;
+  return v < q ? v : v - q;
+}
+
+__device__ __forceinline__ int64_t center(int64_t v, int64_t twice_q) {
+  const int64_t q = twice_q >> 1;
+  return v <= (q >> 1) ? v : v - q;
+}
+
+__device__ __forceinline__ int64_t shift_positive(int64_t v, int64_t twice_q) {
+  return v + (twice_q >> 1);
+}
+
+// ---------------------------------------------------------------------------
+// NTT arithmetic.  A butterfly needs V = w*x mod q in canonical form.  Any
+// algorithm that returns the canonical product gives the reference's value,
+// because the reference REDC also canonicalizes; the lazy add/sub around it
+// are reproduced exactly.  Two exact algorithms are used, chosen per prime:
+//
+//  * q < 2^46 ("small" rows, the 40/41-bit scale primes): binary64.  gfx950
+//    issues FP64 FMA at the full VALU rate, while 32x32->64 integer
+//    multiplies are quarter rate, so this path is several times cheaper.
+//    With x < 2q and w < q the product is split error-free as h + l (FMA),
+//    the quotient estimate t = round(h/q) is off by less than 0.6, and
+//    r = (h - t q) + l is computed exactly, giving r in (-q, q).
+//  * otherwise (the 60-bit base and P primes): Shoup multiplication with the
+//    precomputed w' = floor(w 2^64 / q), r = w x - hi64(w' x) q in [0, 2q).
+// ---------------------------------------------------------------------------
+constexpr double kRoundMagic = 6755399441055744.0;  // 1.5 * 2^52
+
+__device__ __forceinline__ double f64_mulmod(double w, double x, double q,
+                                             double qinv) {
+  const double h = w * x;
+  const double l = __builtin_fma(w, x, -h);
+  const double t = __builtin_fma(h, qinv, kRoundMagic) - kRoundMagic;
+  const double r = __builtin_fma(-t, q, h) + l;
+  return r < 0.0 ? r + q : r;
+}
+
+__device__ __forceinline__ uint64_t shoup_mulmod(uint64_t w, uint64_t wp,
+                                                 uint64_t x, uint64_t q) {
+  const uint64_t t = __umul64hi(wp, x);
+  const uint64_t r = w * x - t * q;
+  return r >= q ? r - q : r;
+}
+
+// Exact u64 <-> f64 conversions for values below 2^53.
+__device__ __forceinline__ double u64_to_f64(uint64_t v) {
+  const uint32_t lo = static_cast<uint32_t>(v);
+  const uint32_t hi = static_cast<uint32_t>(v >> 32);
+  return __builtin_fma(static_cast<double>(hi), 4294967296.0,
+                       static_cast<double>(lo));
+}
+
+__device__ __forceinline__ uint64_t f64_to_u64(double v) {
+  const uint32_t hi = static_cast<uint32_t>(v * 2.3283064365386963e-10);
+  const double rest = __builtin_fma(-static_cast<double>(hi), 4294967296.0, v);
+  const uint32_t lo = static_cast<uint32_t>(rest);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Per-row constants of the gfx950 NTT backend, one 128-byte record per QP
+// row, built on the host (fhelium_amd/ntt.py) and kept in HBM.
+struct NttRow {
+  uint64_t q;
+  uint64_t twice_q;
+  double qd;          // q as binary64 (small rows)
+  double qinv;        // 1/q rounded (small rows)
+  uint64_t small;     // 1: binary64 path, 0: Shoup path
+  uint64_t to_mont;   // R mod q        (x -> x R)
+  uint64_t to_mont_p; // Shoup companion
+  uint64_t ninv_mont;    // N^{-1} mod q           (Montgomery output)
+  uint64_t ninv_mont_p;
+  uint64_t ninv_std;     // N^{-1} R^{-1} mod q     (standard output)
+  uint64_t ninv_std_p;
+  uint64_t reserved[5];
+};
+static_assert(sizeof(NttRow) == 128, "NttRow must stay 128 bytes");
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace fh

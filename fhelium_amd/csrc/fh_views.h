@@ -1,0 +1,56 @@
+// Strided operand views shared by the element-wise kernels and the C ABI.
+#pragma once
+
+#include <cstdint>
+
+extern "C" {
+
+// A rank-3 [batch, limb, coefficient] operand.  Strides are in elements.  A
+// batch stride of 0 broadcasts a single batch entry (the only broadcast the
+// reference accepts, csrc/ops/common/rns_batch.h:42-68).
+typedef struct fh_view3 {
+  int64_t* data;
+  int64_t stride_batch;
+  int64_t stride_limb;
+  int64_t stride_coef;
+} fh_view3;
+
+// A [parameter, limb] table such as rns_params (rns_parameters.h:6-16).
+typedef struct fh_table2 {
+  const int64_t* data;
+  int64_t stride_row;
+  int64_t stride_col;
+} fh_table2;
+
+// A rank-1 vector with a stride (row scalars, twice moduli, tables).
+typedef struct fh_vec {
+  const int64_t* data;
+  int64_t stride;
+} fh_vec;
+
+}  // extern "C"
+
+#ifdef __HIPCC__
+namespace fh {
+
+struct View3 {
+  int64_t* p;
+  int64_t s0, s1, s2;
+  __host__ __device__ View3() : p(nullptr), s0(0), s1(0), s2(0) {}
+  __host__ __device__ explicit View3(const fh_view3& v)
+      : p(v.data), s0(v.stride_batch), s1(v.stride_limb), s2(v.stride_coef) {}
+  __device__ __forceinline__ int64_t& at(int64_t b, int64_t l, int64_t i) const {
+    return p[b * s0 + l * s1 + i * s2];
+  }
+};
+
+struct Vec {
+  const int64_t* p;
+  int64_t s;
+  __host__ __device__ Vec() : p(nullptr), s(0) {}
+  __host__ __device__ explicit Vec(const fh_vec& v) : p(v.data), s(v.stride) {}
+  __device__ __forceinline__ int64_t operator[](int64_t i) const { return p[i * s]; }
+};
+
+}  // namespace fh
+#endif

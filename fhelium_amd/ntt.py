@@ -1,0 +1,123 @@
+"""gfx950 NTT backend.
+
+Implements the reference's ``NttBackend`` protocol
+(fhelium/engine/ntt/interface.py:23-115): representation-explicit forward /
+inverse transforms over dense ``[*batch, limb, N]`` operands whose limbs are a
+contiguous interval of the canonical QP rows starting at
+``parameter_row_start``.
+
+The backend owns its table layout (the protocol leaves layout to backends):
+per QP row a 128-byte record (modulus, binary64 reciprocal, conversion
+constants) plus standard-form twiddles in the compact bit-reversed order
+``psi^{br(k)}``; rows of primes below 2^46 keep the twiddle as a binary64
+bit pattern, larger primes keep ``(w, floor(w 2^64 / q))`` Shoup pairs.  The
+tables are derived on the device from the reference-format inputs
+(``rns_params`` and compact Montgomery twiddles) by ``fh_ntt_prepare_tables``.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from fhelium_amd import _native as nat
+
+NTT_ROW_WORDS = 16  # sizeof(NttRow) / 8
+
+# inverse output forms (fh_ntt_inverse)
+OUT_MONTGOMERY = 0
+OUT_STANDARD_LAZY = 1
+OUT_STANDARD = 2
+OUT_CENTERED = 3
+
+
+class NttTables:
+    """Device tables for one direction pair over a set of QP rows."""
+
+    def __init__(self, rns_params: torch.Tensor, forward_mont: torch.Tensor,
+                 inverse_mont: torch.Tensor, log_n: int) -> None:
+        nat.require_device(rns_params, forward_mont, inverse_mont)
+        rows = rns_params.size(1)
+        n = 1 << log_n
+        if forward_mont.shape != (rows, n) or inverse_mont.shape != (rows, n):
+            raise ValueError("twiddle tables must be [rows, N] matching rns_params")
+        dev = rns_params.device
+        self.log_n = log_n
+        self.rows = torch.zeros((rows, NTT_ROW_WORDS), dtype=torch.int64, device=dev)
+        self.fwd = torch.empty((rows, n), dtype=torch.int64, device=dev)
+        self.fwd_shoup = torch.empty((rows, n), dtype=torch.int64, device=dev)
+        self.inv = torch.empty((rows, n), dtype=torch.int64, device=dev)
+        self.inv_shoup = torch.empty((rows, n), dtype=torch.int64, device=dev)
+        s = nat.stream_ptr(dev)
+        for tw, w, wp in ((forward_mont, self.fwd, self.fwd_shoup), (inverse_mont, self.inv, self.inv_shoup)):
+            nat.call("fh_ntt_prepare_tables", rns_params.data_ptr(), rns_params.stride(0),
+                     rns_params.stride(1), rows, log_n, tw.data_ptr(), tw.stride(0),
+                     self.rows.data_ptr(), w.data_ptr(), wp.data_ptr(), s)
+
+
+MODE_EXACT = 0
+MODE_FAST = 1
+
+
+class Gfx950NttBackend:
+    """The ``NttBackend`` protocol on gfx950 (name ``gfx950_radix16_lds``).
+
+    ``mode="exact"`` reproduces the reference's lazy forward outputs bit for
+    bit; ``mode="fast"`` returns canonical forward outputs (equal to the
+    reference's modulo q) at a fraction of the arithmetic.  Inverse outputs
+    are identical in both modes.
+    """
+
+    name = "gfx950_radix16_lds"
+
+    def __init__(self, tables: NttTables, mode: str = "fast") -> None:
+        if mode not in ("fast", "exact"):
+            raise ValueError("mode must be 'fast' or 'exact'")
+        self.tables = tables
+        self.mode = mode
+        self._mode = MODE_FAST if mode == "fast" else MODE_EXACT
+
+    def _run(self, fn: str, operand: torch.Tensor, row_start: int, mode: int) -> None:
+        nat.require_device(operand)
+        if operand.dtype != torch.int64:
+            raise TypeError("the gfx950 backend supports int64 residues")
+        t = self.tables
+        if operand.size(-1) != (1 << t.log_n):
+            raise ValueError(f"operand ring dimension {operand.size(-1)} != {1 << t.log_n}")
+        if row_start < 0 or row_start + operand.size(-2) > t.rows.size(0):
+            raise ValueError(
+                "NTT operand rows exceed the prepared canonical parameter range: "
+                f"start={row_start}, rows={operand.size(-2)}, table_rows={t.rows.size(0)}"
+            )
+        view = nat.batch_view(operand, "residues")
+        if view.stride(2) != 1:
+            raise ValueError("NTT operands need a unit coefficient stride")
+        fwd = fn == "fh_ntt_forward"
+        nat.call(fn, view.data_ptr(), view.size(0), view.size(1), t.log_n, view.stride(0),
+                 view.stride(1), t.rows.data_ptr(),
+                 (t.fwd if fwd else t.inv).data_ptr(),
+                 (t.fwd_shoup if fwd else t.inv_shoup).data_ptr(),
+                 row_start, mode, self._mode, nat.stream_ptr(operand.device))
+
+    # -- protocol --------------------------------------------------------
+    def forward_montgomery_(self, operand: torch.Tensor, parameter_row_start: int) -> None:
+        self._run("fh_ntt_forward", operand, parameter_row_start, 0)
+
+    def forward_to_montgomery_(self, operand: torch.Tensor, parameter_row_start: int) -> None:
+        self._run("fh_ntt_forward", operand, parameter_row_start, 1)
+
+    def forward_to_montgomery(self, operand: torch.Tensor, parameter_row_start: int) -> torch.Tensor:
+        out = operand.clone(memory_format=torch.contiguous_format)
+        self.forward_to_montgomery_(out, parameter_row_start)
+        return out
+
+    def inverse_montgomery_(self, operand: torch.Tensor, parameter_row_start: int) -> None:
+        self._run("fh_ntt_inverse", operand, parameter_row_start, OUT_MONTGOMERY)
+
+    def inverse_to_standard_lazy_(self, operand: torch.Tensor, parameter_row_start: int) -> None:
+        self._run("fh_ntt_inverse", operand, parameter_row_start, OUT_STANDARD_LAZY)
+
+    def inverse_to_standard_(self, operand: torch.Tensor, parameter_row_start: int) -> None:
+        self._run("fh_ntt_inverse", operand, parameter_row_start, OUT_STANDARD)
+
+    def inverse_to_centered_(self, operand: torch.Tensor, parameter_row_start: int) -> None:
+        self._run("fh_ntt_inverse", operand, parameter_row_start, OUT_CENTERED)

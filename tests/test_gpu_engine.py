@@ -1,0 +1,151 @@
+"""CkksEngine on gfx950: bit-exact parity with the oracle's restatement of
+the reference engine methods, and decrypted error against an FP64 cleartext.
+
+Parity (logN=12 over the full L34 chain, 39 QP rows, identical keys and
+inputs): multiply, relinearize, rescale_to_next_level, rotate_by_step and
+hoisted rotations reproduce the reference CPU residues exactly; the NTT
+conversion is exact in exact mode and equal modulo q (canonical) in fast
+mode.  Error (logN=16, the L34 preset): decrypted max-abs error of
+HMult / rotation vs binary64 stays within the reference's tolerance.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from fhelium_amd.engine import CkksEngine
+from fhelium_amd.core import MaximumLevelError, CkksStateError
+from oracle.engine_ref import EngineRef
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+# reference tolerances at scale 2^40 (tests/test_ckks_operation_correctness.py:48-51)
+TOL_KEYSWITCH = 2e-5
+TOL_MULTIPLY = 3e-5
+
+
+@pytest.fixture(scope="module", params=["fast", "exact"])
+def small(request):
+    cfg = H.config(12)
+    eng = CkksEngine(cfg, ntt_mode=request.param, seed=7)
+    eng.generate_secret_key()
+    eng.generate_relinearization_key()
+    return eng, EngineRef(H.host_tables(12)), request.param
+
+
+def _np(t):
+    return H.to_np(t)
+
+
+def _fresh(eng, level, batch=(2,), seed=0):
+    rng = np.random.default_rng(seed)
+    v = rng.uniform(-1, 1, size=batch + (eng.config.slots,))
+    return eng.encrypt(v, level=level), v
+
+
+@pytest.mark.parametrize("level", [0, 13, 31])
+def test_hmult_chain_bit_exact(small, level):
+    eng, ref, mode = small
+    a, _ = _fresh(eng, level, seed=1)
+    b, _ = _fresh(eng, level, seed=2)
+    na, nb = eng.coefficient_domain_to_ntt_domain(a), eng.coefficient_domain_to_ntt_domain(b)
+    ra = ref.coefficient_domain_to_ntt_domain(_np(a.data), level)
+    rb = ref.coefficient_domain_to_ntt_domain(_np(b.data), level)
+    q = np.array(eng.config.moduli, dtype=np.int64)[level: eng.config.num_q_primes].reshape(1, 1, -1, 1)
+    if mode == "exact":
+        np.testing.assert_array_equal(_np(na.data), ra)
+    else:
+        np.testing.assert_array_equal(_np(na.data), ra % q)
+    prod = eng.multiply(na, nb)
+    rprod = ref.multiply(ra, rb, level)
+    np.testing.assert_array_equal(_np(prod.data), rprod)  # canonical products: mode-independent
+    relin = eng.relinearize(prod)
+    rrelin = ref.relinearize(rprod, _np(eng.relinearization_key.data), level)
+    np.testing.assert_array_equal(_np(relin.data), rrelin)
+    resc = eng.rescale_to_next_level(relin)
+    np.testing.assert_array_equal(_np(resc.data), ref.rescale(rrelin, level))
+    assert resc.level == level + 1
+
+
+@pytest.mark.parametrize("step", [1, -7, 513])
+def test_rotation_bit_exact(small, step):
+    eng, ref, _ = small
+    level = 3
+    a, _ = _fresh(eng, level, seed=3)
+    out = eng.rotate_by_step(a, step)
+    key = eng.rotation_key(step)
+    want = ref.rotate(_np(a.data), _np(key.data), key.galois_element, level)
+    np.testing.assert_array_equal(_np(out.data), want)
+    hoisted = eng.rotate_many_with_keys(a, [key, eng.rotation_key(2)])
+    np.testing.assert_array_equal(_np(hoisted[0].data), want)
+    np.testing.assert_array_equal(_np(hoisted[1].data), _np(eng.rotate_by_step(a, 2).data))
+
+
+def test_state_errors(small):
+    eng, _, _ = small
+    a, _ = _fresh(eng, 0, batch=(1,))
+    with pytest.raises(CkksStateError):
+        eng.multiply(a, a)  # coefficient-domain inputs are rejected
+    last = eng.config.public_level_count - 1
+    b, _ = _fresh(eng, last, batch=(1,))
+    with pytest.raises(MaximumLevelError):
+        eng.rescale_to_next_level(b)
+
+
+@pytest.fixture(scope="module")
+def full():
+    eng = CkksEngine(H.L34, ntt_mode="fast", seed=11)
+    eng.generate_secret_key()
+    eng.generate_relinearization_key()
+    return eng
+
+
+def test_full_size_hmult_error(full):
+    eng = full
+    rng = np.random.default_rng(20260807)
+    x = rng.uniform(-1, 1, eng.config.slots)
+    y = rng.uniform(-1, 1, eng.config.slots)
+    cx, cy = eng.encrypt(x), eng.encrypt(y)
+    out = eng.hmult(cx, cy)
+    got = eng.decrypt(out)
+    err = np.abs(got.real - x * y).max()
+    print(f"logN=16 L34 HMult max-abs error {err:.3e}")
+    assert err < TOL_MULTIPLY
+
+
+def test_full_size_rotation_error(full):
+    eng = full
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, eng.config.slots)
+    c = eng.encrypt(x, level=2)
+    out = eng.rotate_by_step(c, 1)
+    got = eng.decrypt(out)
+    err = np.abs(got.real - np.roll(x, 1)).max()
+    print(f"logN=16 L34 rotate(1) max-abs error {err:.3e}")
+    assert err < TOL_KEYSWITCH
+
+
+def test_basic_flow_l7():
+    """README flow on preset slots8192-scale40-levels7 (logN=14): add, NTT,
+    multiply, relinearize, rescale, rotate by 1, decrypt (atol 2e-5)."""
+
+    eng = CkksEngine(H.L7, seed=0x12345678)
+    eng.generate_secret_key()
+    x = np.linspace(-0.05, 0.05, 32)
+    y = np.random.default_rng(20260807).uniform(-0.05, 0.05, 32)
+    cx, cy = eng.encrypt(x), eng.encrypt(y)
+    # addition of coefficient/standard ciphertexts (canonical add per component)
+    csum = cx.clone()
+    csum.data = torch.stack([eng.runtime.add_canonical(cx.data[i], cy.data[i], 0) for i in range(2)])
+    s = eng.decrypt(csum)[:32].real
+    np.testing.assert_allclose(s, x + y, atol=2e-5)
+    prod = eng.rescale_to_next_level(eng.relinearize(eng.multiply(eng.coefficient_domain_to_ntt_domain(cx),
+                                                                  eng.coefficient_domain_to_ntt_domain(cy))))
+    p = eng.decrypt(prod)[:32].real
+    np.testing.assert_allclose(p, x * y, atol=2e-5)
+    rot = eng.rotate_by_step(prod, 1)
+    r = eng.decrypt(rot).real
+    np.testing.assert_allclose(r[1:33], (x * y), atol=2e-5)

@@ -1,0 +1,111 @@
+"""gfx950 NTT / inverse NTT vs the CPU oracle, bit for bit.
+
+The oracle restates the reference's indexed radix-2 CPU transform
+(csrc/ops/ntt/cpu/indexed_ntt_cpu.cpp).  In exact mode the GPU kernels must
+reproduce its lazy [0, 2q) forward outputs exactly (not merely mod q); in
+fast mode the forward output must be exactly the canonical representative
+of the oracle's value.  Inverse outputs are canonical in the reference and
+must match bit for bit in both modes.  Rows cover the 40/41-bit primes
+(binary64 path) and the 60-bit base/P primes (integer path).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from fhelium_amd import ntt as gntt
+from fhelium_amd import rns
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _backend(log_n: int, mode: str = "exact"):
+    cfg = H.config(log_n)
+    t = H.host_tables(log_n)
+    fwd, inv = rns.compact_twiddles_montgomery(cfg.moduli, log_n)
+    params = t.rns_params
+    tables = gntt.NttTables(params.cuda(), fwd.cuda(), inv.cuda(), log_n)
+    return cfg, params.numpy(), fwd.numpy(), inv.numpy(), gntt.Gfx950NttBackend(tables, mode)
+
+
+def _canonical(x: np.ndarray, moduli: np.ndarray) -> np.ndarray:
+    return x % moduli[None, :, None]
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("log_n", [4, 9, 11, 12, 13, 14, 15, 16])
+@pytest.mark.parametrize("to_mont", [False, True])
+def test_forward_bit_exact(log_n, to_mont, mode):
+    cfg, params, fwd, _, be = _backend(log_n, mode)
+    rng = np.random.default_rng(1000 + log_n)
+    n = cfg.N
+    # all 39 QP rows: 35 binary64 rows + 1 base + 4 P (integer path)
+    x = H.random_residues(rng, 2, H.moduli_array(cfg), n, lazy=not to_mont)
+    dev = H.to_dev(x)
+    if to_mont:
+        be.forward_to_montgomery_(dev, 0)
+    else:
+        be.forward_montgomery_(dev, 0)
+    want = O.ntt_forward(x, fwd, params, to_montgomery=to_mont)
+    if mode == "fast":
+        want = _canonical(want, H.moduli_array(cfg))
+    np.testing.assert_array_equal(H.to_np(dev), want)
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("log_n", [4, 9, 11, 12, 14, 16])
+@pytest.mark.parametrize("out_form", [0, 1, 2, 3])
+def test_inverse_bit_exact(log_n, out_form, mode):
+    cfg, params, _, inv, be = _backend(log_n, mode)
+    rng = np.random.default_rng(2000 + log_n * 4 + out_form)
+    x = H.random_residues(rng, 2, H.moduli_array(cfg), cfg.N, lazy=True)
+    dev = H.to_dev(x)
+    {0: be.inverse_montgomery_, 1: be.inverse_to_standard_lazy_, 2: be.inverse_to_standard_,
+     3: be.inverse_to_centered_}[out_form](dev, 0)
+    want = O.ntt_inverse(x, inv, params, out_form)
+    np.testing.assert_array_equal(H.to_np(dev), want)
+
+
+@pytest.mark.parametrize("chunk_rows", [0, 5, 128])
+def test_row_start_and_strided_batch(chunk_rows):
+    """A level-5 QP interval (rows 5..38) inside a strided batch view, with
+    the two-pass schedule split into chunks that cut across batch entries."""
+    from fhelium_amd import _native as nat
+    log_n = 13
+    cfg, params, fwd, inv, be = _backend(log_n)
+    nat.call("fh_ntt_set_chunk_rows", chunk_rows)
+    rng = np.random.default_rng(7)
+    start = 5
+    mods = H.moduli_array(cfg, start)
+    x = H.random_residues(rng, 3, mods, cfg.N)
+    big = torch.zeros((3, 2, len(mods), cfg.N), dtype=torch.int64, device="cuda")
+    big[:, 1] = H.to_dev(x)
+    view = big[:, 1]
+    be.forward_to_montgomery_(view, start)
+    sub = slice(start, None)
+    want = O.ntt_forward(x, np.ascontiguousarray(fwd[sub]), np.ascontiguousarray(params[:, sub]), True)
+    np.testing.assert_array_equal(H.to_np(view), want)
+    assert int(big[:, 0].abs().sum()) == 0
+    be.inverse_to_standard_(view, start)
+    back = O.ntt_inverse(want, np.ascontiguousarray(inv[sub]), np.ascontiguousarray(params[:, sub]), 2)
+    np.testing.assert_array_equal(H.to_np(view), back)
+    np.testing.assert_array_equal(back, x)  # to-Montgomery then standard inverse is the identity
+    nat.call("fh_ntt_set_chunk_rows", 128)
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_round_trip_full_size(mode):
+    """[4, 2, 35, 2^16] slice of the ring-kernel workload: inverse(forward(x)) == x."""
+    cfg, params, _, _, be = _backend(16, mode)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    q = torch.tensor(cfg.moduli[:35], dtype=torch.int64, device="cuda").view(1, 1, 35, 1)
+    x = (torch.randint(0, 2**62, (4, 2, 35, cfg.N), generator=g, device="cuda") % q)
+    y = x.clone()
+    be.forward_to_montgomery_(y, 0)
+    assert bool((y < (2 * q if mode == "exact" else q)).all())
+    be.inverse_to_standard_(y, 0)
+    assert torch.equal(x, y)
