@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""Headline benchmark: HMult(relin+rescale)/s + rotations/s at logN=16, L=34,
+batched vs plain, with p50/p99 latency.
+
+Workload (uneven batch): C ciphertext pairs (default 256) with levels
+l = 34 - d, d ~ U{2..34} (so every rescale is legal, l in 0..32), slots
+uniform[-1, 1] at scale 2^40; each pair does HMult (NTT conversion of both
+inputs, multiply, relinearize, rescale) and then a rotation by its own step
+s ~ U{1..1024} with that step's direct key (all keys HBM-resident).
+  * plain:   one ciphertext per call sequence;
+  * batched: ciphertexts of equal level processed as one batched call
+             (shared relinearization key, per-entry rotation keys).
+One "step" is one pass over all C pairs; value = (HMults + rotations) per
+second in batched mode.  Inputs are encrypted and resident in HBM before
+timing.  Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+import torch
+
+METRIC = "rotations/s + HMult(relin+rescale)/s at logN=16 L=34, batched vs plain; p99"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse_args():
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--count", type=int, default=256, help="ciphertext pairs per step")
+    p.add_argument("--seed", type=int, default=20260807)
+    p.add_argument("--ntt-mode", default="fast", choices=["fast", "exact"])
+    p.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
+    p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    return p.parse_args()
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(eng, count: int, seed: int):
+    rng = np.random.default_rng(seed)
+    levels = 34 - rng.integers(2, 35, size=count)
+    steps = rng.integers(1, 1025, size=count)
+    slots = eng.config.slots
+    groups = {}
+    for lvl in sorted(set(levels.tolist())):
+        idx = np.nonzero(levels == lvl)[0]
+        x = rng.uniform(-1, 1, size=(len(idx), slots))
+        y = rng.uniform(-1, 1, size=(len(idx), slots))
+        groups[lvl] = {
+            "index": idx,
+            "steps": [int(steps[i]) for i in idx],
+            "a": eng.encrypt(x, level=lvl),
+            "b": eng.encrypt(y, level=lvl),
+            "x": x,
+            "y": y,
+        }
+    return levels, steps, groups
+
+
+def entry(ct, j):
+    out = ct.clone()
+    out.data = ct.data[:, j: j + 1]
+    return out
+
+
+def run_plain(eng, groups, lat_h, lat_r):
+    outs = []
+    for lvl, g in groups.items():
+        for j, s in enumerate(g["steps"]):
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record()
+            h = eng.hmult(entry(g["a"], j), entry(g["b"], j))
+            e1.record()
+            r = eng.rotate_by_step(h, s)
+            e2.record()
+            if lat_h is not None:
+                lat_h.append((e0, e1))
+                lat_r.append((e1, e2))
+            outs.append(r)
+    return outs
+
+
+def run_batched(eng, groups, lat_h, lat_r):
+    outs = []
+    for lvl, g in groups.items():
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        h = eng.hmult(g["a"], g["b"])
+        e1.record()
+        r = eng.rotate_by_steps(h, g["steps"])
+        e2.record()
+        if lat_h is not None:
+            lat_h.append((e0, e1, len(g["steps"])))
+            lat_r.append((e1, e2, len(g["steps"])))
+        outs.append(r)
+    return outs
+
+
+def timed(fn, steps: int, warmup: int):
+    for _ in range(warmup):
+        fn(None, None)
+    torch.cuda.synchronize()
+    lat_h, lat_r = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn(lat_h, lat_r)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1000.0 / steps, lat_h, lat_r
+
+
+def percentile(values, q):
+    return float(np.percentile(np.asarray(values), q)) if values else None
+
+
+def cpu_baseline(eng, seconds_budget: float = 30.0):
+    """The oracle (CPU restatement of the reference kernels) on the host's
+    cores: one HMult + one rotation at level 16 of the same preset."""
+
+    from oracle.engine_ref import EngineRef
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    os.environ["FH_ORACLE_THREADS"] = str(threads)
+    ref = EngineRef(eng.runtime.host)
+    level = 16
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, size=(1, eng.config.slots))
+    a = eng.encrypt(x, level=level)
+    b = eng.encrypt(x, level=level)
+    rlk = eng.relinearization_key.data.cpu().numpy()
+    rot = eng.rotation_key(1)
+    rotk = rot.data.cpu().numpy()
+    an, bn = a.data.cpu().numpy(), b.data.cpu().numpy()
+    ops = 0
+    t0 = time.perf_counter()
+    while True:
+        na = ref.coefficient_domain_to_ntt_domain(an, level)
+        nb = ref.coefficient_domain_to_ntt_domain(bn, level)
+        r = ref.rescale(ref.relinearize(ref.multiply(na, nb, level), rlk, level), level)
+        ref.rotate(r, rotk, rot.galois_element, level + 1)
+        ops += 2
+        if time.perf_counter() - t0 > seconds_budget / 3 or ops >= 6:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": ops / dt, "unit": "ops/s", "cores": threads, "kind": "port",
+            "sample": f"{ops // 2} x (HMult at level {level} + rotation at level {level + 1}), "
+                      "logN=16 L34, oracle C restatement with OpenMP over RNS rows"}
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see the sharding note in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        return 2
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm device", file=sys.stderr)
+        return 2
+    torch.cuda.set_device(0)
+    from fhelium_amd import _native as nat
+    from fhelium_amd.config import Preset
+    from fhelium_amd.engine import CkksEngine
+
+    log(nat.device_info())
+    t_setup = time.perf_counter()
+    eng = CkksEngine(Preset.slots32768_scale40_levels34_int64, ntt_mode=args.ntt_mode, seed=args.seed)
+    eng.generate_secret_key()
+    eng.generate_relinearization_key()
+    levels, steps, groups = build_workload(eng, args.count, args.seed)
+    for i, s in enumerate(sorted(set(steps.tolist()))):
+        eng.generate_rotation_key(int(s))
+        if i % 50 == 49:
+            torch.cuda.synchronize()
+            log(f"{i + 1} rotation keys generated")
+    torch.cuda.synchronize()
+    log(f"setup {time.perf_counter() - t_setup:.1f}s: {args.count} pairs, {len(groups)} levels, "
+        f"{len(eng.rotation_keys)} rotation keys, {torch.cuda.memory_allocated() / 2**30:.1f} GiB resident")
+
+    # correctness sample (outside timing): decrypt a few batched results
+    outs = run_batched(eng, groups, None, None)
+    max_err = 0.0
+    for (lvl, g), out in list(zip(groups.items(), outs))[:3]:
+        got = eng.decrypt(entry(out, 0)).real[0]
+        want = np.roll(g["x"][0] * g["y"][0], g["steps"][0])
+        max_err = max(max_err, float(np.abs(got - want).max()))
+    log(f"decrypted max-abs error (sample) {max_err:.3e}")
+
+    ms_plain, plh, plr = timed(lambda h, r: run_plain(eng, groups, h, r), args.steps, args.warmup)
+    log(f"plain   {ms_plain:.2f} ms/step")
+    eng.runtime.ntt.recorder = []
+    ms_batch, bh, br = timed(lambda h, r: run_batched(eng, groups, h, r), args.steps, args.warmup)
+    rec = eng.runtime.ntt.recorder
+    eng.runtime.ntt.recorder = None
+    log(f"batched {ms_batch:.2f} ms/step")
+
+    n_ops = 2 * args.count
+    plain_h = [a.elapsed_time(b) for a, b in plh]
+    plain_r = [a.elapsed_time(b) for a, b in plr]
+    batch_h = [a.elapsed_time(b) for a, b, _ in bh]
+    batch_r = [a.elapsed_time(b) for a, b, _ in br]
+    ntt_ms = sum(a.elapsed_time(b) for a, b, _ in rec)
+    ntt_bytes = sum(nb for _, _, nb in rec)
+    ntt_calls = len(rec) // args.steps if args.steps else 0
+    achieved = ntt_bytes / (ntt_ms * 1e-3) / 1e9 if ntt_ms > 0 else None
+
+    result = {
+        "metric": METRIC,
+        "value": n_ops / (ms_batch * 1e-3),
+        "unit": "ops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic: seeded uniform[-1,1] slot vectors encrypted at scale 2^40, random keys",
+        "config": {
+            "workload": "batched uneven (BASELINE configs[3]): {} ciphertext pairs, levels 0..32, "
+                        "HMult+relin+rescale then rotate by per-pair step 1..1024".format(args.count),
+            "logN": 16, "levels": 34, "pairs": args.count, "ntt_mode": args.ntt_mode,
+        },
+        "hmult_per_s": args.count / (sum(batch_h) / args.steps * 1e-3),
+        "rotations_per_s": args.count / (sum(batch_r) / args.steps * 1e-3),
+        "plain": {
+            "value": n_ops / (ms_plain * 1e-3),
+            "ms_per_step": ms_plain,
+            "hmult_per_s": args.count / (sum(plain_h) / args.steps * 1e-3),
+            "rotations_per_s": args.count / (sum(plain_r) / args.steps * 1e-3),
+            "hmult_p50_ms": percentile(plain_h, 50), "hmult_p99_ms": percentile(plain_h, 99),
+            "rotation_p50_ms": percentile(plain_r, 50), "rotation_p99_ms": percentile(plain_r, 99),
+        },
+        "batched_vs_plain": ms_plain / ms_batch,
+        "p50_ms": percentile(batch_h + batch_r, 50),
+        "p99_ms": percentile(batch_h + batch_r, 99),
+        "latency_note": "batched p50/p99 are per batched call (one level group, HMult or rotation phase)",
+        "max_abs_error": max_err,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "NTT/INTT (fh_ntt_forward / fh_ntt_inverse, two-pass radix-16)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "calls_per_step": ntt_calls,
+            "algorithmic_bytes_per_call": (ntt_bytes / len(rec)) if rec else None,
+        },
+    }
+    if args.cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        result["cpu_baseline"] = cpu_baseline(eng)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

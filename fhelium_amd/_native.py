@@ -54,6 +54,7 @@ SIGNATURES: dict[str, list] = {
     "fh_ntt_forward": [_ptr, _i64, _i64, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i32, _i32, _ptr],
     "fh_ntt_inverse": [_ptr, _i64, _i64, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i32, _i32, _ptr],
     "fh_ntt_set_chunk_rows": [_i64],
+    "fh_ntt_indexed_stage": [fh_view3, _i64, _i64, _ptr, _ptr, _i64, _ptr, _i64, _i64, fh_table2, _i64, _i32, _ptr],
     "fh_ntt_prepare_tables": [_ptr, _i64, _i64, _i64, _i64, _ptr, _i64, _ptr, _ptr, _ptr, _ptr],
     "fh_rns_binary": [_i32, fh_view3, fh_view3, fh_view3, fh_table2, _i64, _i64, _i64, _i64, _ptr],
     "fh_rns_unary_inplace": [_i32, fh_view3, fh_table2, _i64, _i64, _i64, _ptr],
@@ -71,6 +72,8 @@ SIGNATURES: dict[str, list] = {
     "fh_ckks_moddown": [fh_view3, fh_view3, fh_view3, fh_table2, fh_table2, _i64, _i64, _i64, _i64, _ptr],
     "fh_ckks_add_plaintext": [_i32, fh_view3, fh_view3, fh_view3, _ptr, _i64, fh_table2, _i64, _i64, _i64, _i64, _ptr],
     "fh_ckks_tensor_product": [fh_view3] * 7 + [fh_table2, _i64, _i64, _i64, _ptr],
+    "fh_ckks_keyswitch_accumulate_multi_key": [fh_view3, fh_view3, fh_view3, _ptr, _i64, _i64, _i64, _i64,
+                                               fh_table2, _i64, _i64, _i64, _i64, _ptr],
 }
 
 _lock = threading.Lock()

@@ -15,6 +15,7 @@ import sys
 
 PACKAGE = pathlib.Path(__file__).resolve().parent
 CSRC = PACKAGE / "csrc"
+INCLUDE = PACKAGE.parent / "include"
 LIB_DIR = PACKAGE / "_lib"
 LIB_PATH = LIB_DIR / "libfhelium_amd.so"
 OFFLOAD_ARCH = "gfx950"
@@ -59,7 +60,7 @@ def build(verbose: bool = False) -> pathlib.Path:
 
     LIB_DIR.mkdir(exist_ok=True)
     tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{CSRC}", *map(str, sources()), "-o", str(tmp)]
+    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{CSRC}", f"-I{INCLUDE}", *map(str, sources()), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)

@@ -91,6 +91,32 @@ __global__ __launch_bounds__(kThreads) void k_keyswitch_mac(View3 acc0, View3 ac
   }
 }
 
+// Same multiply-accumulate with a different key per batch entry: entry b
+// reads key digit `digit_index` of the key whose base address is
+// key_bases[b] (each key [digit, 2, QP, N] with the given element strides).
+// Used when a batch of ciphertexts rotates by per-ciphertext steps.
+__global__ __launch_bounds__(kThreads) void k_keyswitch_mac_multi(
+    View3 acc0, View3 acc1, View3 digit, const int64_t* key_bases, int64_t digit_index,
+    int64_t key_stride_digit, int64_t key_stride_comp, int64_t key_stride_row, ParamView params,
+    int64_t key_row_start, int64_t n) {
+  const int64_t l = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const Mont m = load_mont(params, l);
+  const int64_t* key = reinterpret_cast<const int64_t*>(key_bases[b]) +
+                       digit_index * key_stride_digit + (key_row_start + l) * key_stride_row;
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kChunk + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int64_t i = base + e * kThreads;
+    if (i >= n) break;
+    const int64_t x = digit.at(b, l, i);
+    const int64_t p0 = mont_mul(x, key[i], m);
+    const int64_t p1 = mont_mul(x, key[key_stride_comp + i], m);
+    acc0.at(b, l, i) = add_lazy(acc0.at(b, l, i), p0, m.twice_q);
+    acc1.at(b, l, i) = add_lazy(acc1.at(b, l, i), p1, m.twice_q);
+  }
+}
+
 // keyswitch_moddown_qp_to_q (ckks_cpu.cpp:474-601): floor(x / P) mod q_i by
 // sequentially dropping canonical P residues, one coefficient per thread.
 constexpr int kMaxP = 8;
@@ -351,5 +377,26 @@ FH_API int fh_ckks_tensor_product(fh_view3 d0, fh_view3 d1, fh_view3 d2, fh_view
   hipLaunchKernelGGL(k_tensor_product, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream),
                      View3(d0), View3(d1), View3(d2), View3(a0), View3(a1), View3(b0), View3(b1), pv, n);
   FH_CHECK_LAUNCH("ckks tensor product");
+  return kOk;
+}
+
+FH_API int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view3 digit,
+                                                  const int64_t* key_bases, int64_t digit_index,
+                                                  int64_t key_stride_digit, int64_t key_stride_comp,
+                                                  int64_t key_stride_row, fh_table2 rns_params,
+                                                  int64_t key_row_start, int64_t batch,
+                                                  int64_t limbs, int64_t n, void* stream) {
+  int st = check_grid(batch, limbs, n);
+  if (st != kOk) return st;
+  if (batch == 0 || limbs == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(acc0.data && acc1.data && digit.data && key_bases && rns_params.data,
+               "multi-key accumulate: null operand");
+  FH_CHECK_ARG(key_row_start >= 0 && digit_index >= 0, "negative key row or digit index");
+  ParamView pv{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  dim3 grid(ceil_div(n, kChunk), limbs, batch);
+  hipLaunchKernelGGL(k_keyswitch_mac_multi, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream),
+                     View3(acc0), View3(acc1), View3(digit), key_bases, digit_index, key_stride_digit,
+                     key_stride_comp, key_stride_row, pv, key_row_start, n);
+  FH_CHECK_LAUNCH("ckks multi-key keyswitch accumulate");
   return kOk;
 }

@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cstring>
 
+#include "fhelium_amd.h"  // every definition below is checked against it
+
 #define FH_API extern "C" __attribute__((visibility("default")))
 
 namespace fh {

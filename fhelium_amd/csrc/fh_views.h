@@ -3,32 +3,8 @@
 
 #include <cstdint>
 
-extern "C" {
 
-// A rank-3 [batch, limb, coefficient] operand.  Strides are in elements.  A
-// batch stride of 0 broadcasts a single batch entry (the only broadcast the
-// reference accepts, csrc/ops/common/rns_batch.h:42-68).
-typedef struct fh_view3 {
-  int64_t* data;
-  int64_t stride_batch;
-  int64_t stride_limb;
-  int64_t stride_coef;
-} fh_view3;
-
-// A [parameter, limb] table such as rns_params (rns_parameters.h:6-16).
-typedef struct fh_table2 {
-  const int64_t* data;
-  int64_t stride_row;
-  int64_t stride_col;
-} fh_table2;
-
-// A rank-1 vector with a stride (row scalars, twice moduli, tables).
-typedef struct fh_vec {
-  const int64_t* data;
-  int64_t stride;
-} fh_vec;
-
-}  // extern "C"
+#include "fhelium_amd.h"  // fh_view3, fh_table2, fh_vec and every entry point
 
 #ifdef __HIPCC__
 namespace fh {

@@ -666,6 +666,40 @@ int launch(NttArgs a, int64_t total_rows, bool forward, int mode, hipStream_t st
 }
 
 // ---------------------------------------------------------------------------
+// Indexed schedule (the reference CPU backend's family,
+// indexed_ntt_cpu.cpp:96-170, 206-258): one launch per stage, one butterfly
+// per thread, explicit even/odd index tables and expanded Montgomery
+// twiddles [limb, stage, butterfly]; REDC exactly as the CPU does it.
+// Grid: (ceil(butterflies / 256), limbs, batch).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_indexed_stage(
+    int64_t* data, int64_t batch_stride, int64_t limb_stride, int64_t coef_stride,
+    const int32_t* even, const int32_t* odd, int64_t idx_stride, const int64_t* tw,
+    int64_t tw_limb_stride, int64_t tw_bfly_stride, ParamView params, int64_t butterflies,
+    int forward) {
+  const int64_t bf = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (bf >= butterflies) return;
+  const int64_t l = blockIdx.y;
+  const int64_t b = blockIdx.z;
+  const Mont m = load_mont(params, l);
+  int64_t* row = data + b * batch_stride + l * limb_stride;
+  const int64_t e = even[bf * idx_stride];
+  const int64_t o = odd[bf * idx_stride];
+  const int64_t w = tw[l * tw_limb_stride + bf * tw_bfly_stride];
+  const int64_t u = row[e * coef_stride];
+  const int64_t v = row[o * coef_stride];
+  if (forward) {
+    const int64_t p = mont_mul(w, v, m);
+    row[e * coef_stride] = add_lazy(u, p, m.twice_q);
+    row[o * coef_stride] = sub_lazy(u, p, m.twice_q);
+  } else {
+    const int64_t d = sub_lazy(u, v, m.twice_q);
+    row[e * coef_stride] = add_lazy(u, v, m.twice_q);
+    row[o * coef_stride] = mont_mul(w, d, m);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Table preparation: derive the backend's per-row records and standard-form
 // twiddles from the reference-format inputs (rns_params [8, limb] and compact
 // Montgomery twiddles [limb, N], fhelium/engine/ntt/plans/twiddles.py:45-84).
@@ -797,5 +831,32 @@ FH_API int fh_ntt_prepare_tables(const int64_t* rns_params, int64_t param_row_st
                        twiddles_shoup_out);
     FH_CHECK_LAUNCH("ntt prepare twiddles");
   }
+  return kOk;
+}
+
+// One stage of an indexed radix-2 schedule.  even/odd point at that stage's
+// [butterflies] index rows; twiddles at that stage's [limb, butterflies]
+// Montgomery slice.  The caller validates that the schedule partitions the
+// coefficients (indexed_ntt_cpu.cpp:315-354) before the first stage.
+FH_API int fh_ntt_indexed_stage(fh_view3 residues, int64_t batch, int64_t limbs,
+                                const int32_t* even_indices, const int32_t* odd_indices,
+                                int64_t index_stride, const int64_t* stage_twiddles,
+                                int64_t twiddle_limb_stride, int64_t twiddle_butterfly_stride,
+                                fh_table2 rns_params, int64_t butterflies, int32_t forward,
+                                void* stream) {
+  FH_CHECK_ARG(batch >= 0 && limbs >= 0 && butterflies >= 0, "negative extent");
+  FH_CHECK_ARG(batch <= 65535 && limbs <= 65535, "batch or limb extent exceeds the launch grid");
+  if (batch == 0 || limbs == 0 || butterflies == 0) return kOk;
+  FH_CHECK_ARG(residues.data && even_indices && odd_indices && stage_twiddles && rns_params.data,
+               "indexed stage: null operand");
+  ParamView pv{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  hipLaunchKernelGGL(k_indexed_stage,
+                     dim3(static_cast<uint32_t>(ceil_div(butterflies, kBlock)),
+                          static_cast<uint32_t>(limbs), static_cast<uint32_t>(batch)),
+                     dim3(kBlock), 0, static_cast<hipStream_t>(stream), residues.data,
+                     residues.stride_batch, residues.stride_limb, residues.stride_coef, even_indices,
+                     odd_indices, index_stride, stage_twiddles, twiddle_limb_stride,
+                     twiddle_butterfly_stride, pv, butterflies, forward);
+  FH_CHECK_LAUNCH("ntt indexed stage");
   return kOk;
 }

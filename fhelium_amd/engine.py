@@ -38,7 +38,9 @@ class CkksEngine:
         self.layout = self.runtime.layout
         self.key_switcher = HybridKeySwitcher(self.runtime)
         self.allow_secret_key_generation = allow_secret_key_generation
-        self._gen = torch.Generator(device="cpu").manual_seed(seed)
+        # Seeded device sampler (Philox); key and ciphertext randomness never
+        # leaves HBM.
+        self._gen = torch.Generator(device=self.device).manual_seed(seed)
         self.secret_key: SecretKey | None = None
         self.relinearization_key: RelinearizationKey | None = None
         self.rotation_keys: dict[int, RotationKey] = {}
@@ -48,17 +50,19 @@ class CkksEngine:
     # Sampling (this module's own seeded sampler)
     # ------------------------------------------------------------------
     def _uniform_qp(self, shape_prefix: tuple[int, ...], rows: tuple[int, ...]) -> torch.Tensor:
-        q = torch.tensor([self.config.moduli[i] for i in rows], dtype=torch.int64)
+        q = torch.tensor([self.config.moduli[i] for i in rows], dtype=torch.int64, device=self.device)
         raw = torch.randint(0, 2**62, (*shape_prefix, len(rows), self.config.N), generator=self._gen,
-                            dtype=torch.int64)
-        return (raw % q.view(*([1] * len(shape_prefix)), -1, 1)).to(self.device)
+                            dtype=torch.int64, device=self.device)
+        return raw % q.view(*([1] * len(shape_prefix)), -1, 1)
 
     def _gaussian(self, shape_prefix: tuple[int, ...]) -> torch.Tensor:
-        e = torch.randn((*shape_prefix, self.config.N), generator=self._gen, dtype=torch.float64)
+        e = torch.randn((*shape_prefix, self.config.N), generator=self._gen, dtype=torch.float64,
+                        device=self.device)
         return torch.round(e * self.config.sigma).to(torch.int64)
 
     def _ternary(self) -> torch.Tensor:
-        return torch.randint(-1, 2, (self.config.N,), generator=self._gen, dtype=torch.int64)
+        return torch.randint(-1, 2, (self.config.N,), generator=self._gen, dtype=torch.int64,
+                             device=self.device)
 
     def _lift_signed(self, coefficients: torch.Tensor, rows: tuple[int, ...]) -> torch.Tensor:
         """Signed integer coefficients [*, N] -> canonical residues [*, rows, N] on device."""
@@ -124,7 +128,7 @@ class CkksEngine:
         sk = self._require_secret()
         step = RotationKey.canonical_step(rotation_step, cfg.N)
         g = rns.rotation_galois_element(cfg.N, step, self.galois_generator)
-        idx, sgn = rns.coefficient_galois_tables(cfg.N, g)
+        idx, sgn = self.runtime.coefficient_galois(g)
         s_rot = sk.coefficients[idx.long()] * sgn.to(torch.int64)
         src = self._to_ntt_rows(self._lift_signed(s_rot, tuple(range(cfg.total_num_primes))), 0)
         key = RotationKey(self.create_key_switch_key(src), rotation_step=step, galois_element=g)
@@ -292,11 +296,40 @@ class CkksEngine:
         out[1] = corr1
         return Ciphertext(out, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
 
-    def rotate_many_with_keys(self, ct: Ciphertext, keys: list[RotationKey]) -> list[Ciphertext]:
-        """Hoisted rotations: one ModUp + NTT of c1 shared by every key
-        (ckks_engine.py:2298-2331, 3767-3820)."""
+    def rotate_by_steps(self, ct: Ciphertext, rotation_steps: list[int]) -> Ciphertext:
+        """Rotate batch entry b of a batched ciphertext [2, B, limb, N] by
+        rotation_steps[b] in one batched key switch (each entry uses its own
+        rotation key; no reference counterpart -- the reference batches only
+        homogeneous operations)."""
 
         self._assert_rotation_ciphertext(ct)
+        if len(ct.batch_shape) != 1 or ct.batch_shape[0] != len(rotation_steps):
+            raise ValueError("rotate_by_steps needs a [2, B, limb, N] ciphertext and B steps")
+        rt = self.runtime
+        keys = [self.rotation_key(s) for s in rotation_steps]
+        twice = rt.twice_modulus[rt.row_start(ct.level): self.config.num_q_primes]
+        rotated = torch.empty_like(ct.data)
+        for b, key in enumerate(keys):
+            idx, sgn = rt.coefficient_galois(key.galois_element)
+            rotated[:, b] = K.apply_coefficient_galois_automorphism(ct.data[:, b], idx, sgn, twice)
+        corr0, corr1 = self.key_switcher.apply_key_switch(rotated[1], keys, ct.level)
+        out = torch.empty_like(rotated)
+        out[0] = K.add_canonical(rotated[0], corr0, rt.params(ct.level))
+        out[1] = corr1
+        return Ciphertext(out, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
+
+    def rotate_many_with_keys(self, ct: Ciphertext, keys: list[RotationKey], *,
+                              use_hoisting: bool = True) -> list[Ciphertext]:
+        """Hoisted rotations: one ModUp + NTT of c1 shared by every key
+        (ckks_engine.py:2298-2331, 3767-3820).  As in the reference, fewer
+        than two non-zero steps fall back to direct rotations."""
+
+        self._assert_rotation_ciphertext(ct)
+        for key in keys:
+            if RotationKey.canonical_step(key.rotation_step, self.config.N) != key.rotation_step:
+                raise ValueError("rotation key must carry a canonical signed step")
+        if not use_hoisting or sum(k.rotation_step != 0 for k in keys) < 2:
+            return [ct.clone() if k.rotation_step == 0 else self.rotate_with_key(ct, k) for k in keys]
         rt = self.runtime
         prepared = self.key_switcher.prepare_rotation_digits(ct.c1, ct.level)
         twice = rt.twice_modulus[rt.row_start(ct.level): self.config.num_q_primes]

@@ -378,6 +378,37 @@ def keyswitch_accumulate_digit_products_(accumulator0_qp, accumulator1_qp, exten
              nat.stream_ptr(d.device))
 
 
+def keyswitch_accumulate_multi_key_(accumulator0_qp, accumulator1_qp, extended_digit_ntt_qp,
+                                    keys: list[torch.Tensor], key_bases: torch.Tensor,
+                                    digit_index: int, rns_params, key_digit_row_start):
+    """keyswitch_accumulate_digit_products_ with keys[b] for batch entry b.
+
+    ``keys`` are same-shaped [digit, 2, QP, N] tensors (kept alive by the
+    caller); ``key_bases`` is the int64 device tensor of their addresses."""
+
+    op = "keyswitch_accumulate_multi_key"
+    _peers(op, extended_digit_ntt_qp, accumulator0_qp, accumulator1_qp, rns_params)
+    a0 = nat.batch_view(accumulator0_qp, "accumulator0_qp")
+    a1 = nat.batch_view(accumulator1_qp, "accumulator1_qp")
+    d = nat.batch_view(extended_digit_ntt_qp, "digit_qp")
+    if a0.shape != d.shape or a1.shape != d.shape or len(keys) != d.size(0):
+        raise RuntimeError(f"{op}: one key per batch entry and matching accumulator shapes required")
+    _check_params(op, d.size(1), rns_params)
+    k = keys[0]
+    for other in keys:
+        if other.shape != k.shape or other.stride() != k.stride() or other.device != k.device:
+            raise RuntimeError(f"{op}: keys must share shape, strides and device")
+    if (k.dim() != 4 or k.size(1) != 2 or not 0 <= digit_index < k.size(0)
+            or key_digit_row_start + d.size(1) > k.size(2) or k.size(3) != d.size(2)):
+        raise RuntimeError(f"{op} key shape or row interval mismatch")
+    if key_bases.dtype != torch.int64 or key_bases.numel() != d.size(0) or key_bases.device != d.device:
+        raise RuntimeError(f"{op}: key_bases must be an int64 device vector of length batch")
+    nat.call("fh_ckks_keyswitch_accumulate_multi_key", nat.v3(a0), nat.v3(a1), nat.v3(d),
+             key_bases.data_ptr(), digit_index, k.stride(0), k.stride(1), k.stride(2),
+             nat.t2(rns_params), int(key_digit_row_start), d.size(0), d.size(1), d.size(2),
+             nat.stream_ptr(d.device))
+
+
 def _add_plaintext(opname, layout, ciphertext_component, plaintext, rns_params, out, implicit=None):
     _peers(opname, ciphertext_component, plaintext, rns_params)
     ct = nat.batch_view(ciphertext_component, "ciphertext_component")

@@ -42,13 +42,23 @@ class HybridKeySwitcher:
         return K.mixed_radix_basis_extend_to_montgomery(
             digits, t.basis_extension[:, start:], qp_params, qp_params.size(1))
 
-    def apply_key_switch(self, component: torch.Tensor, key: KeySwitchKey,
+    def apply_key_switch(self, component: torch.Tensor, key: KeySwitchKey | list[KeySwitchKey],
                          level: int) -> tuple[torch.Tensor, torch.Tensor]:
-        """``component``: [*batch, Q_level, N] coefficient/standard (canonical)."""
+        """``component``: [*batch, Q_level, N] coefficient/standard (canonical).
+
+        ``key`` is one key shared by the batch, or a list with one key per
+        batch entry (batched rotations by per-ciphertext steps)."""
 
         rt = self.rt
         start = rt.row_start(level)
         qp_params = rt.params(level, include_p=True)
+        per_entry = isinstance(key, (list, tuple))
+        if per_entry:
+            flat = component.reshape(-1, component.size(-2), component.size(-1))
+            if len(key) != flat.size(0):
+                raise ValueError("one key per batch entry is required")
+            keys = [k.data for k in key]
+            bases = torch.tensor([k.data_ptr() for k in keys], dtype=torch.int64, device=component.device)
         acc0 = acc1 = None
         for spec in self.layout.digit_specs(level):
             digit_qp = self._mod_up(component, spec, level)
@@ -56,8 +66,12 @@ class HybridKeySwitcher:
                 acc0 = torch.zeros_like(digit_qp)
                 acc1 = torch.zeros_like(digit_qp)
             rt.ntt.forward_montgomery_(digit_qp, start)
-            K.keyswitch_accumulate_digit_products_(acc0, acc1, digit_qp, key.digit(spec.key_digit_index),
-                                                   qp_params, start)
+            if per_entry:
+                K.keyswitch_accumulate_multi_key_(acc0, acc1, digit_qp, keys, bases, spec.key_digit_index,
+                                                  qp_params, start)
+            else:
+                K.keyswitch_accumulate_digit_products_(acc0, acc1, digit_qp,
+                                                       key.digit(spec.key_digit_index), qp_params, start)
         return self.mod_down(acc0, acc1, level)
 
     def prepare_rotation_digits(self, component: torch.Tensor, level: int) -> torch.Tensor:

@@ -75,6 +75,9 @@ class Gfx950NttBackend:
         self.tables = tables
         self.mode = mode
         self._mode = MODE_FAST if mode == "fast" else MODE_EXACT
+        # Optional list receiving (start_event, end_event, algorithmic_bytes)
+        # per transform, recorded on the operand's stream (roofline probes).
+        self.recorder: list | None = None
 
     def _run(self, fn: str, operand: torch.Tensor, row_start: int, mode: int) -> None:
         nat.require_device(operand)
@@ -92,11 +95,20 @@ class Gfx950NttBackend:
         if view.stride(2) != 1:
             raise ValueError("NTT operands need a unit coefficient stride")
         fwd = fn == "fh_ntt_forward"
+        rec = self.recorder
+        if rec is not None:
+            start = torch.cuda.Event(enable_timing=True)
+            end = torch.cuda.Event(enable_timing=True)
+            start.record()
         nat.call(fn, view.data_ptr(), view.size(0), view.size(1), t.log_n, view.stride(0),
                  view.stride(1), t.rows.data_ptr(),
                  (t.fwd if fwd else t.inv).data_ptr(),
                  (t.fwd_shoup if fwd else t.inv_shoup).data_ptr(),
                  row_start, mode, self._mode, nat.stream_ptr(operand.device))
+        if rec is not None:
+            end.record()
+            # algorithmic bytes: every residue read once and written once
+            rec.append((start, end, 2 * view.size(0) * view.size(1) * view.size(2) * 8))
 
     # -- protocol --------------------------------------------------------
     def forward_montgomery_(self, operand: torch.Tensor, parameter_row_start: int) -> None:

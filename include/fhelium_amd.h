@@ -1,0 +1,173 @@
+/*
+ * fhelium_amd -- C ABI of the gfx950 CKKS hot-path library (libfhelium_amd.so).
+ *
+ * Every entry point takes raw device pointers, element strides and sizes plus
+ * a hipStream_t (passed as void*), launches asynchronously on that stream and
+ * returns 0 on success or a negative status (fh_last_error() gives the text,
+ * per host thread).  Nothing here allocates device memory.  Residues are
+ * int64; an operand is a [batch, limb, coefficient] view (fh_view3) whose
+ * limb j is parameter column j of the accompanying rns_params table, exactly
+ * as in the reference's native ops (csrc/ops/common/rns_batch.h:11-116,
+ * rns_parameters.h:6-16).  Each function names the reference operator
+ * schema(s) it implements; the PyTorch shim (fhelium_amd/native) registers
+ * them under those unchanged schemas for ROCm tensors.
+ */
+#ifndef FHELIUM_AMD_H_
+#define FHELIUM_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* [batch, limb, coefficient] operand; a batch stride of 0 broadcasts one
+ * batch entry (the reference's only broadcast, rns_batch.h:42-68). */
+typedef struct fh_view3 {
+  int64_t* data;
+  int64_t stride_batch;
+  int64_t stride_limb;
+  int64_t stride_coef;
+} fh_view3;
+
+/* [parameter, limb] table, e.g. rns_params (rns_parameters.h:6-16). */
+typedef struct fh_table2 {
+  const int64_t* data;
+  int64_t stride_row;
+  int64_t stride_col;
+} fh_table2;
+
+typedef struct fh_vec {
+  const int64_t* data;
+  int64_t stride;
+} fh_vec;
+
+/* ---- runtime (replaces csrc/runtime/cuda_info.{h,cpp} inspection) ---- */
+const char* fh_last_error(void);
+int fh_abi_version(void);
+int fh_device_info(char* buf, int64_t buf_len);
+
+/* ---- NTT (csrc/ops/ntt/forward_ntt.cpp:25-80, inverse_ntt.cpp:23-82) ----
+ * Backend tables: ntt_rows is one 128-byte record per QP row and
+ * twiddles / twiddles_shoup are [rows, N], all produced by
+ * fh_ntt_prepare_tables from the reference-format rns_params and compact
+ * Montgomery twiddles (fhelium/engine/ntt/plans/twiddles.py:45-84).
+ * Operand limb j uses table row row_start + j (ntt/interface.py:118-176).
+ * mode 0 = exact (reference lazy outputs bit for bit), 1 = fast (canonical
+ * forward outputs, equal modulo q; inverse outputs identical). */
+int fh_ntt_prepare_tables(const int64_t* rns_params, int64_t param_row_stride,
+                          int64_t param_limb_stride, int64_t limbs, int64_t log_n,
+                          const int64_t* montgomery_twiddles, int64_t twiddle_row_stride,
+                          void* ntt_rows_out, int64_t* twiddles_out,
+                          int64_t* twiddles_shoup_out, void* stream);
+/* forward_ntt_{montgomery,to_montgomery}_compact_grouped_smem_ and the
+ * indexed / power-of-two families: to_montgomery = 1 consumes standard
+ * residues (x -> xR first). */
+int fh_ntt_forward(int64_t* residues, int64_t batch, int64_t limbs, int64_t log_n,
+                   int64_t batch_stride, int64_t limb_stride, const void* ntt_rows,
+                   const int64_t* twiddles, const int64_t* twiddles_shoup, int64_t row_start,
+                   int32_t to_montgomery, int32_t mode, void* stream);
+/* inverse_ntt_{montgomery,to_standard_lazy,to_standard,to_centered}_*:
+ * output_form 0..3 in that order. */
+int fh_ntt_inverse(int64_t* residues, int64_t batch, int64_t limbs, int64_t log_n,
+                   int64_t batch_stride, int64_t limb_stride, const void* ntt_rows,
+                   const int64_t* twiddles, const int64_t* twiddles_shoup, int64_t row_start,
+                   int32_t output_form, int32_t mode, void* stream);
+/* One stage of forward/inverse_ntt_*_indexed_ (indexed_ntt_cpu.cpp:96-170,
+ * 206-258) with explicit even/odd index rows and the stage's expanded
+ * Montgomery twiddles; the caller validates the schedule first. */
+int fh_ntt_indexed_stage(fh_view3 residues, int64_t batch, int64_t limbs,
+                         const int32_t* even_indices, const int32_t* odd_indices,
+                         int64_t index_stride, const int64_t* stage_twiddles,
+                         int64_t twiddle_limb_stride, int64_t twiddle_butterfly_stride,
+                         fh_table2 rns_params, int64_t butterflies, int32_t forward, void* stream);
+/* Tuning knob (no reference counterpart; the reference's smem-budget
+ * diagnostic ops play this role, forward_ntt.cpp:71-80): rows per launch of
+ * the two-pass schedule, 0 = as many as the grid allows. */
+int fh_ntt_set_chunk_rows(int64_t rows);
+
+/* ---- RNS arithmetic (csrc/ops/rns/rns_arithmetic.cpp:30-72,
+ *      rns_canonical_arithmetic.cpp:15-23) ----
+ * op: 0 montgomery_mul, 1 add_lazy, 2 sub_lazy, 3 add_canonical(_),
+ *     4 sub_canonical(_), 5 montgomery_mul_cyclic_compressed,
+ *     6 montgomery_mul_contiguous_compressed (rhs has compressed_support
+ *     coefficients). */
+int fh_rns_binary(int32_t op, fh_view3 out, fh_view3 lhs, fh_view3 rhs, fh_table2 rns_params,
+                  int64_t batch, int64_t limbs, int64_t n, int64_t compressed_support,
+                  void* stream);
+/* op: 0 to_montgomery_, 1 from_montgomery_, 2 canonicalize_residues_,
+ *     3 center_residues_, 4 shift_residues_positive_. */
+int fh_rns_unary_inplace(int32_t op, fh_view3 residues, fh_table2 rns_params, int64_t batch,
+                         int64_t limbs, int64_t n, void* stream);
+/* montgomery_mul_row_scalars_ (canonical = 0) and
+ * montgomery_mul_row_scalars_canonical (canonical = 1). */
+int fh_rns_row_scalars(fh_view3 out, fh_view3 in, fh_vec scalars, fh_table2 rns_params,
+                       int64_t batch, int64_t limbs, int64_t n, int32_t canonical, void* stream);
+/* add_lazy_with_twice_modulus */
+int fh_rns_add_lazy_twice_modulus(fh_view3 out, fh_view3 lhs, fh_view3 rhs, fh_vec twice_modulus,
+                                  int64_t batch, int64_t limbs, int64_t n, void* stream);
+/* lift_centered_coefficients: coefficients is [batch, 1, N] (limb stride 0). */
+int fh_rns_lift_centered(fh_view3 out, fh_view3 coefficients, fh_vec twice_modulus, int64_t batch,
+                         int64_t limbs, int64_t n, void* stream);
+/* mixed_radix_decompose (rows <= 8) */
+int fh_rns_mixed_radix_decompose(fh_view3 out, fh_view3 source, int64_t batch, int64_t rows,
+                                 int64_t n, fh_vec normalizers, fh_table2 propagation,
+                                 fh_vec modulus_lo, fh_vec modulus_hi, fh_vec neg_inv_lo,
+                                 fh_vec neg_inv_hi, void* stream);
+/* mixed_radix_basis_extend_to_montgomery */
+int fh_rns_basis_extend(fh_view3 out, fh_view3 digits, int64_t batch, int64_t digit_count,
+                        int64_t destination_rows, int64_t n, fh_table2 coefficients,
+                        fh_table2 rns_params, void* stream);
+
+/* ---- CKKS (csrc/ops/ckks/ckks.cpp:29-90) ---- */
+/* rescale_drop_leading_prime_{nearest,truncate}(_): dropped is
+ * [batch, 1, N] (limb stride 0). */
+int fh_ckks_rescale(fh_view3 out, fh_view3 remaining, fh_view3 dropped,
+                    fh_vec drop_prime_inverse_mont, fh_table2 rns_params, int64_t batch,
+                    int64_t limbs, int64_t n, int64_t half_drop_prime, int32_t nearest,
+                    void* stream);
+/* apply_coefficient_galois_automorphism (indices must lie in [0, N)) */
+int fh_ckks_galois_coefficient(fh_view3 out, fh_view3 in, const int32_t* source_indices,
+                               int64_t index_stride, const int8_t* source_sign,
+                               int64_t sign_stride, fh_vec twice_modulus, int64_t batch,
+                               int64_t limbs, int64_t n, void* stream);
+/* apply_ntt_galois_automorphism (indices must lie in [0, N)) */
+int fh_ckks_galois_ntt(fh_view3 out, fh_view3 in, const int32_t* source_indices,
+                       int64_t index_stride, int64_t batch, int64_t limbs, int64_t n,
+                       void* stream);
+/* keyswitch_accumulate_digit_products_: key_digit is [2, key rows, N]. */
+int fh_ckks_keyswitch_accumulate(fh_view3 acc0, fh_view3 acc1, fh_view3 digit,
+                                 fh_view3 key_digit, fh_table2 rns_params, int64_t key_row_start,
+                                 int64_t batch, int64_t limbs, int64_t n, void* stream);
+/* keyswitch_accumulate_digit_products_ with one key per batch entry:
+ * key_bases is a device array of [batch] key base addresses (each key
+ * [digit, 2, QP, N] with the given element strides); entry b uses digit
+ * digit_index of key_bases[b].  No reference counterpart: it lets a batch
+ * of ciphertexts rotate by per-ciphertext steps in one launch. */
+int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view3 digit,
+                                           const int64_t* key_bases, int64_t digit_index,
+                                           int64_t key_stride_digit, int64_t key_stride_comp,
+                                           int64_t key_stride_row, fh_table2 rns_params,
+                                           int64_t key_row_start, int64_t batch, int64_t limbs,
+                                           int64_t n, void* stream);
+/* keyswitch_moddown_qp_to_q */
+int fh_ckks_moddown(fh_view3 out, fh_view3 q_residues, fh_view3 p_residues,
+                    fh_table2 p_drop_inverses, fh_table2 rns_params, int64_t batch,
+                    int64_t q_count, int64_t p_count, int64_t n, void* stream);
+/* add_{prepared,cyclic_compressed,contiguous_compressed,strided}_plaintext_component(_):
+ * layout 0..3 in that order. */
+int fh_ckks_add_plaintext(int32_t layout, fh_view3 out, fh_view3 ciphertext, fh_view3 plaintext,
+                          const int64_t* implicit_plaintext, int64_t plaintext_batch_count,
+                          fh_table2 rns_params, int64_t batch, int64_t limbs, int64_t n,
+                          int64_t support, void* stream);
+/* CkksEngine.multiply's five native calls as one pass
+ * (fhelium/engine/ckks_engine.py:1583-1598). */
+int fh_ckks_tensor_product(fh_view3 d0, fh_view3 d1, fh_view3 d2, fh_view3 a0, fh_view3 a1,
+                           fh_view3 b0, fh_view3 b1, fh_table2 rns_params, int64_t batch,
+                           int64_t limbs, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FHELIUM_AMD_H_ */

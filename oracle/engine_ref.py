@@ -135,6 +135,45 @@ class EngineRef:
             outs.append(r.reshape(*comp.shape[:-2], comp.shape[-2] - 1, comp.shape[-1]))
         return np.stack(outs)
 
+    def prepare_rotation_digits(self, c1, level):
+        """hybrid_keyswitch.py:373-417: ModUp + forward NTT of every digit."""
+
+        qp_p = self._p(level, True)
+        out = []
+        for spec in self.t.layout.digit_specs(level):
+            rows = spec.component_row_ids
+            src = np.ascontiguousarray(c1[:, rows[0]: rows[-1] + 1])
+            d = self.t.digit_tables[spec.prime_ids]
+            if len(rows) > 1:
+                digits = O.mixed_radix_decompose(src, d.normalizers.numpy(), d.propagation.numpy(),
+                                                 d.modulus_lo.numpy(), d.modulus_hi.numpy(),
+                                                 d.neg_inv_lo.numpy(), d.neg_inv_hi.numpy())
+            else:
+                digits = src
+            coef = np.ascontiguousarray(d.basis_extension.numpy()[:, level:])
+            ext = O.basis_extend(digits, coef, qp_p, qp_p.shape[1])
+            out.append(self.ntt_forward(ext, level, True, to_montgomery=False))
+        return out
+
+    def rotate_hoisted(self, c, prepared, key, galois_element, level):
+        """ckks_engine.py:3786-3820 with hybrid_keyswitch.py:419-504."""
+
+        n = self.cfg.N
+        qp_p = self._p(level, True)
+        src = rns.ntt_galois_indices(self.cfg.logN, galois_element).numpy()
+        acc0 = np.zeros_like(prepared[0])
+        acc1 = np.zeros_like(prepared[0])
+        for spec, digit in zip(self.t.layout.digit_specs(level), prepared):
+            rotated = O.galois_ntt(digit, src)
+            acc0, acc1 = O.keyswitch_mac(acc0, acc1, rotated, key[spec.key_digit_index], qp_p, level)
+        corr0, corr1 = self.mod_down(acc0, acc1, level)
+        idx, sgn = rns.coefficient_galois_tables(n, galois_element)
+        twice = 2 * np.array(self.cfg.moduli, dtype=np.int64)[self._rows(level, False)]
+        flat = lambda x: x.reshape(-1, x.shape[-2], x.shape[-1])
+        r0 = O.galois_coefficient(flat(c[0]), idx.numpy(), sgn.numpy(), twice)
+        out0 = O.binary("add_canonical", r0, corr0, self._p(level))
+        return np.stack([out0.reshape(c.shape[1:]), corr1.reshape(c.shape[1:])])
+
     def rotate(self, c, key, galois_element, level):
         n = self.cfg.N
         idx, sgn = rns.coefficient_galois_tables(n, galois_element)
