@@ -39,6 +39,7 @@ def parse_args():
     p.add_argument("--count", type=int, default=256, help="ciphertext pairs per step")
     p.add_argument("--seed", type=int, default=20260807)
     p.add_argument("--ntt-mode", default="fast", choices=["fast", "exact"])
+    p.add_argument("--skip-plain", action="store_true", help="time only the batched mode (profiling)")
     p.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     return p.parse_args()
@@ -197,7 +198,10 @@ def main() -> int:
         max_err = max(max_err, float(np.abs(got - want).max()))
     log(f"decrypted max-abs error (sample) {max_err:.3e}")
 
-    ms_plain, plh, plr = timed(lambda h, r: run_plain(eng, groups, h, r), args.steps, args.warmup)
+    if args.skip_plain:
+        ms_plain, plh, plr = float("nan"), [], []
+    else:
+        ms_plain, plh, plr = timed(lambda h, r: run_plain(eng, groups, h, r), args.steps, args.warmup)
     log(f"plain   {ms_plain:.2f} ms/step")
     eng.runtime.ntt.recorder = []
     ms_batch, bh, br = timed(lambda h, r: run_batched(eng, groups, h, r), args.steps, args.warmup)
@@ -235,7 +239,7 @@ def main() -> int:
         },
         "hmult_per_s": args.count / (sum(batch_h) / args.steps * 1e-3),
         "rotations_per_s": args.count / (sum(batch_r) / args.steps * 1e-3),
-        "plain": {
+        "plain": None if args.skip_plain else {
             "value": n_ops / (ms_plain * 1e-3),
             "ms_per_step": ms_plain,
             "hmult_per_s": args.count / (sum(plain_h) / args.steps * 1e-3),
@@ -243,7 +247,7 @@ def main() -> int:
             "hmult_p50_ms": percentile(plain_h, 50), "hmult_p99_ms": percentile(plain_h, 99),
             "rotation_p50_ms": percentile(plain_r, 50), "rotation_p99_ms": percentile(plain_r, 99),
         },
-        "batched_vs_plain": ms_plain / ms_batch,
+        "batched_vs_plain": None if args.skip_plain else ms_plain / ms_batch,
         "p50_ms": percentile(batch_h + batch_r, 50),
         "p99_ms": percentile(batch_h + batch_r, 99),
         "latency_note": "batched p50/p99 are per batched call (one level group, HMult or rotation phase)",

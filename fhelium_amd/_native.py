@@ -74,6 +74,9 @@ SIGNATURES: dict[str, list] = {
     "fh_ckks_tensor_product": [fh_view3] * 7 + [fh_table2, _i64, _i64, _i64, _ptr],
     "fh_ckks_keyswitch_accumulate_multi_key": [fh_view3, fh_view3, fh_view3, _ptr, _i64, _i64, _i64, _i64,
                                                fh_table2, _i64, _i64, _i64, _i64, _ptr],
+    "fh_keyswitch_fused": [fh_view3, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, _i64, _ptr, _ptr, fh_table2, _ptr,
+                           _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
+                           _ptr, fh_view3, fh_view3, fh_view3, fh_view3, _i32, _ptr],
 }
 
 _lock = threading.Lock()

@@ -120,11 +120,27 @@ __device__ __forceinline__ int64_t mont_mul(int64_t a, int64_t b,
   return mont_mul(a, b, m.q, m.k);
 }
 
-// montgomery.h:148-172: signed operands are first reduced with C `%`.
+// montgomery.h:148-172: signed operands are first reduced with C `%` to
+// the canonical residue in [0, q).  64-bit division is a long instruction
+// sequence on the GPU, so for q >= 2^30 the quotient is estimated in
+// binary64 instead: |v/q| < 2^33 and the estimate is within 2^-18 of it,
+// so floor() is off by at most one and r = v - t q lies in [-q, 2q); one
+// correction each way gives the same canonical residue as `%`.
 __device__ __forceinline__ int64_t canonical_operand(int64_t v, uint64_t q) {
-  int64_t c = v % static_cast<int64_t>(q);
-  if (c < 0) c += static_cast<int64_t>(q);
-  return c;
+  const int64_t qs = static_cast<int64_t>(q);
+  if (q < (uint64_t{1} << 30)) {
+    int64_t c = v % qs;
+    if (c < 0) c += qs;
+    return c;
+  }
+  const double qd = static_cast<double>(q);
+  const double est = __builtin_floor(static_cast<double>(v) / qd);
+  // two's-complement wrap-around: the true r is small, so the low 64 bits are it
+  int64_t r = static_cast<int64_t>(static_cast<uint64_t>(v) -
+                                   static_cast<uint64_t>(static_cast<int64_t>(est)) * q);
+  r = r < 0 ? r + qs : r;
+  r = r >= qs ? r - qs : r;
+  return r;
 }
 
 __device__ __forceinline__ int64_t mont_mul_split(int64_t a, int64_t b,
@@ -184,6 +200,15 @@ __device__ __forceinline__ double f64_mulmod(double w, double x, double q,
   return r < 0.0 ? r + q : r;
 }
 
+// The same product without the final correction: w*x - round(w*x/q) q,
+// exact, in (-0.52q, 0.52q) for w < 2q and signed |x| < 2^45.
+__device__ __forceinline__ double f64_mulmod_signed(double w, double x, double q, double qinv) {
+  const double h = w * x;
+  const double l = __builtin_fma(w, x, -h);
+  const double t = __builtin_fma(h, qinv, kRoundMagic) - kRoundMagic;
+  return __builtin_fma(-t, q, h) + l;
+}
+
 __device__ __forceinline__ uint64_t shoup_mulmod(uint64_t w, uint64_t wp,
                                                  uint64_t x, uint64_t q) {
   const uint64_t t = __umul64hi(wp, x);
@@ -220,7 +245,9 @@ struct NttRow {
   uint64_t ninv_mont_p;
   uint64_t ninv_std;     // N^{-1} R^{-1} mod q     (standard output)
   uint64_t ninv_std_p;
-  uint64_t reserved[5];
+  uint64_t ninv_std_rinv;    // N^{-1} R^{-2} mod q (standard output of a
+  uint64_t ninv_std_rinv_p;  // binary64 key MAC that skipped REDC's R^{-1})
+  uint64_t reserved[3];
 };
 static_assert(sizeof(NttRow) == 128, "NttRow must stay 128 bytes");
 

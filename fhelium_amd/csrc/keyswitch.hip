@@ -1,0 +1,713 @@
+// Fused hybrid key switching for gfx950.
+//
+// Same mathematics as the reference's streaming key switch
+// (fhelium/engine/hybrid_keyswitch.py:321-371, 530-568 with the native ops
+// csrc/ops/rns/cpu/rns_arithmetic_cpu.cpp:482-700 and
+// csrc/ops/ckks/cpu/ckks_cpu.cpp:474-708), reorganised into six launches
+// instead of ~100:
+//
+//   1. k_ks_decompose     Garner digits of every active hybrid digit (one
+//                         launch for all digits); optionally reads its input
+//                         through a rotation's Galois gather.
+//   2. k_ks_modup_columns basis extension onto the level's QP rows fused into
+//                         the forward NTT column pass (the extended digit is
+//                         never written out in coefficient form).
+//   3. k_ks_tiles_mac     forward NTT tile pass of every digit fused with the
+//                         key multiply-accumulate; both accumulators stay in
+//                         registers across all digits and are written once.
+//                         Blocks are remapped so one XCD walks the batch
+//                         entries of one (row, tile) consecutively and re-reads
+//                         the key tile from its L2.
+//   4. inverse NTT of both accumulators (ntt.hip, working-format input).
+//   5. k_ks_moddown_add   floor(x / P) mod q_i computed as
+//                         (x - (x mod P)) * P^{-1} with x mod P from the
+//                         mixed-radix digits of the P residues (once per
+//                         coefficient), fused with the engine's final
+//                         canonical addition (optionally through a Galois
+//                         gather of the base).
+//
+// Exactness: digits are the unique mixed-radix digits; the extended digit is
+// formed with the reference's lazy formula (each term a canonical product,
+// computed in binary64 where operands allow); the NTT output is the
+// reference value (exact mode) or its canonical representative (fast mode);
+// each key product is congruent to the reference's REDC product, so the
+// accumulators are congruent to the reference's; the inverse NTT yields the
+// unique canonical values, and ModDown (an exact floor division) and the
+// final canonical addition are exact functions of those.  The outputs are
+// therefore bit-identical to the reference CPU backend's in both modes.
+#include <type_traits>
+
+#include "fh_common.h"
+#include "fh_views.h"
+#include "ntt_core.h"
+
+namespace fh {
+namespace {
+
+using namespace ::fh::ntt;
+
+constexpr int kMaxDigitRows = 8;
+constexpr int kMaxP = 8;
+constexpr int kModDownRowsPerThread = 4;
+
+// One active hybrid digit (RnsDigitSpec plus device table addresses),
+// packed by the host as 16 int64 words.  Also used for the P primes.
+struct KsDigit {
+  int64_t row_begin;     // first Q row of the digit, relative to the level
+  int64_t rows;          // D
+  int64_t key_digit;     // stable level-0 key digit index
+  const int64_t* normalizers;  // [D-1]
+  const int64_t* propagation;  // [D-1, D] row-major
+  const int64_t* lo;           // split moduli / -q^{-1} words [D]
+  const int64_t* hi;
+  const int64_t* klo;
+  const int64_t* khi;
+  const int64_t* basis;        // [D-1, QP0] M_r R^2 mod p_j, row-major
+  int64_t basis_stride;        // QP0
+  const int64_t* basis_std;    // [D-1, QP0] M_r R mod p_j
+  int64_t f64_ok;              // every source prime < 2^45: digits stored as binary64
+  const int64_t* basis_shoup;  // [D-1, QP0] Shoup companions of basis_std
+  const double* moddown_f64;   // P digit only: [2P+1, Q0] binary64 ModDown table
+  int64_t moddown_stride;      // its row stride (Q0)
+};
+static_assert(sizeof(KsDigit) == 16 * 8, "KsDigit must stay 16 words");
+
+struct KsArgs {
+  View3 src;                // component [B, Lq, N], canonical standard
+  int64_t* digits;          // [B, Lq, N] contiguous
+  int64_t* inter;           // [B, dnum, Lqp, N] contiguous (working format)
+  int64_t* acc;             // [2, B, Lqp, N] contiguous (working format)
+  const KsDigit* dig;       // [dnum]
+  int64_t dnum;
+  int64_t batch;
+  int64_t lq, lqp;          // Q rows / QP rows at this level
+  int64_t level;            // global row of limb 0
+  int log_n;
+  int mode;                 // 0 exact, 1 fast
+  ParamView params;         // global QP parameter table [8, QP0]
+  const NttRow* rows;       // global QP NTT records
+  const int64_t* fwd_w;     // global forward twiddles [QP0, N]
+  const int64_t* fwd_wp;
+  const int64_t* key;       // shared key [digit, 2, QP0, N] or null
+  const int64_t* key_bases; // per-entry key addresses [B] or null
+  int64_t key_sd, key_sc, key_sr;   // key strides (digit, component, row)
+  const int64_t* gal_idx;   // per-entry int32 gather table addresses [B] or null
+  const int64_t* gal_sign;  // per-entry int8 sign table addresses [B]
+};
+
+// Table addresses read from device memory are generic pointers to the
+// compiler, which would then emit FLAT loads; those also count against the
+// LDS wait counter and serialise with the LDS exchanges.  Every table lives
+// in global memory, so say so.
+template <class T>
+using Global = const T __attribute__((address_space(1)));
+template <class T>
+__device__ __forceinline__ Global<T>* global(const T* p) {
+  return (Global<T>*)p;
+}
+
+__device__ __forceinline__ Mont digit_mont(const KsDigit& d, int r) {
+  Mont m;
+  m.q = static_cast<uint64_t>(global(d.lo)[r]) + (static_cast<uint64_t>(global(d.hi)[r]) << kHalfRadixBits);
+  m.k = static_cast<uint64_t>(global(d.klo)[r]) + (static_cast<uint64_t>(global(d.khi)[r]) << kHalfRadixBits);
+  m.twice_q = 0;
+  m.r2 = 0;
+  m.ninv = 0;
+  return m;
+}
+
+// Reads component row `row` of batch entry b at coefficient i, through the
+// Galois gather of apply_coefficient_galois_automorphism (ckks_cpu.cpp:390-396)
+// when one is given.
+__device__ __forceinline__ int64_t read_maybe_galois(const View3& v, const int64_t* gal_idx,
+                                                     const int64_t* gal_sign, int64_t b, int64_t row,
+                                                     int64_t i, int64_t twice_q) {
+  if (gal_idx == nullptr) return v.at(b, row, i);
+  Global<int32_t>* idx = global(reinterpret_cast<const int32_t*>(gal_idx[b]));
+  Global<int8_t>* sgn = global(reinterpret_cast<const int8_t*>(gal_sign[b]));
+  int64_t x = v.at(b, row, idx[i]);
+  if (sgn[i] == -1) x = -x;
+  return canonicalize(shift_positive(x, twice_q), twice_q);
+}
+
+// Garner digits of D residues (rns_arithmetic_cpu.cpp:565-600).
+__device__ __forceinline__ void garner(const KsDigit& d, int64_t* dg) {
+  const int rows = static_cast<int>(d.rows);
+  const Mont m0 = digit_mont(d, 0);
+  int64_t first = dg[0];
+  first = first < static_cast<int64_t>(m0.q) ? first : first - static_cast<int64_t>(m0.q);
+  int64_t src[kMaxDigitRows];
+#pragma unroll
+  for (int r = 0; r < kMaxDigitRows; ++r) {
+    src[r] = dg[r];
+    dg[r] = first;
+  }
+#pragma unroll
+  for (int step = 0; step < kMaxDigitRows - 1; ++step) {
+    const int row = step + 1;
+    if (row >= rows) break;
+    const int64_t digit = mont_mul_split(src[row] - dg[row], global(d.normalizers)[step], digit_mont(d, row));
+    dg[row] = digit;
+#pragma unroll
+    for (int t = 0; t < kMaxDigitRows; ++t) {
+      if (t <= row || t >= rows) continue;
+      dg[t] += mont_mul(digit, global(d.propagation)[step * rows + t], digit_mont(d, t));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1. Garner digits.  Grid (N/256, dnum, B).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_ks_decompose(KsArgs a) {
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const KsDigit& d = a.dig[blockIdx.y];
+  const int64_t b = blockIdx.z;
+  const int rows = static_cast<int>(d.rows);
+  int64_t dg[kMaxDigitRows];
+#pragma unroll
+  for (int r = 0; r < kMaxDigitRows; ++r) {
+    dg[r] = r < rows ? read_maybe_galois(a.src, a.gal_idx, a.gal_sign, b, d.row_begin + r, i,
+                                         a.params.at(kTwiceModulus, a.level + d.row_begin + r))
+                     : 0;
+  }
+  garner(d, dg);
+  int64_t* out = a.digits + (b * a.lq + d.row_begin) * n + i;
+  // Digits below 2^45 are stored as binary64 (exact), the form the ModUp
+  // kernel multiplies in; the others stay integers.
+  const bool f64 = d.f64_ok != 0;
+#pragma unroll
+  for (int r = 0; r < kMaxDigitRows; ++r) {
+    if (r < rows)
+      out[static_cast<int64_t>(r) * n] =
+          f64 ? __double_as_longlong(u64_to_f64(static_cast<uint64_t>(dg[r]))) : dg[r];
+  }
+}
+
+// Integer digit value whatever its stored form.
+__device__ __forceinline__ uint64_t digit_u64(const KsDigit& d, int64_t raw) {
+  return d.f64_ok ? f64_to_u64(__longlong_as_double(raw)) : static_cast<uint64_t>(raw);
+}
+
+// Lazy [0, 2q) basis extension of one coefficient onto a QP row by the
+// reference formula (rns_arithmetic_cpu.cpp:676-691): each REDC term is the
+// canonical product d_r (M_r R mod q), computed here as a Shoup product.
+__device__ __forceinline__ uint64_t extend_integer(const KsDigit& d, const NttRow& row, int64_t grow,
+                                                   const int64_t* dig0, int64_t n, int64_t pos) {
+  const int rows = static_cast<int>(d.rows);
+  uint64_t x = shoup_mulmod(row.to_mont, row.to_mont_p, digit_u64(d, dig0[pos]), row.q);
+  for (int r = 1; r < rows; ++r) {
+    const int64_t c = (r - 1) * d.basis_stride + grow;
+    const uint64_t y = shoup_mulmod(static_cast<uint64_t>(global(d.basis_std)[c]),
+                                    static_cast<uint64_t>(global(d.basis_shoup)[c]),
+                                    digit_u64(d, dig0[r * n + pos]), row.q);
+    x += y;
+    x = x >= row.twice_q ? x - row.twice_q : x;
+  }
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// 2. Basis extension + forward NTT column pass.
+//    Grid (T column groups, Lqp destination rows, dnum * B).
+// ---------------------------------------------------------------------------
+template <int S1, class P>
+__device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const KsDigit& d,
+                                              int64_t di, int64_t b, int64_t j, int64_t* lds) {
+  constexpr int R1 = 1 << S1;
+  constexpr int T = R1 / 16;
+  constexpr int C = kBlock / T;
+  using V = typename P::V;
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t grow = a.level + j;  // global QP row of the destination
+  const NttRow& row = a.rows[grow];
+  const int64_t* tw = a.fwd_w + grow * n;
+  const int64_t* twp = a.fwd_wp + grow * n;
+  const int64_t* dig0 = a.digits + (b * a.lq + d.row_begin) * n;
+  const int rows = static_cast<int>(d.rows);
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int col = blockIdx.x * C + c;
+  V v[16];
+  if constexpr (std::is_same<V, double>::value) {
+    if (d.f64_ok) {
+      // Binary64 digits times (M_r R mod q).  Exact arithmetic reproduces
+      // the reference's canonical REDC terms and add_lazy chain; fast
+      // arithmetic keeps the signed congruent sum (|x| < 2.1q), which the
+      // fast butterflies accept.
+      const double q = row.qd, qinv = row.qinv, twoq = 2.0 * row.qd;
+      const double c0 = static_cast<double>(row.to_mont);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const double x = __longlong_as_double(dig0[static_cast<int64_t>(t + T * k) * kTile + col]);
+        v[k] = std::is_same<P, F64Fast>::value ? f64_mulmod_signed(c0, x, q, qinv) : f64_mulmod(c0, x, q, qinv);
+      }
+#pragma unroll
+      for (int r = 1; r < kMaxDigitRows; ++r) {
+        if (r >= rows) break;
+        const double cr = static_cast<double>(global(d.basis_std)[(r - 1) * d.basis_stride + grow]);
+        const int64_t* src = dig0 + r * n;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const double x = __longlong_as_double(src[static_cast<int64_t>(t + T * k) * kTile + col]);
+          if constexpr (std::is_same<P, F64Fast>::value) {
+            v[k] += f64_mulmod_signed(cr, x, q, qinv);
+          } else {
+            const double sum = v[k] + f64_mulmod(cr, x, q, qinv);
+            v[k] = sum >= twoq ? sum - twoq : sum;
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int64_t pos = static_cast<int64_t>(t + T * k) * kTile + col;
+        v[k] = u64_to_f64(extend_integer(d, row, grow, dig0, n, pos));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int64_t pos = static_cast<int64_t>(t + T * k) * kTile + col;
+      v[k] = extend_integer(d, row, grow, dig0, n, pos);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int half = 8 >> s;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      p.fwd(v[k], v[k + half], p.tw(tw, twp, (1 << s) + (k >> (4 - s))));
+    }
+  }
+  int64_t* out = a.inter + ((b * a.dnum + di) * a.lqp + j) * n;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) lds[(t + T * k) * C + c] = p.mid(v[k]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = p.unmid(lds[(16 * t + k) * C + c]);
+#pragma unroll
+  for (int s = 4; s < S1; ++s) {
+    const int half = 1 << (S1 - 1 - s);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & half) continue;
+      p.fwd(v[k], v[k + half], p.tw(tw, twp, (1 << s) + ((16 * t + k) >> (S1 - s))));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) out[static_cast<int64_t>(16 * t + k) * kTile + col] = p.mid(v[k]);
+}
+
+// One launch per arithmetic path, as for the tile pass below.
+template <int S1, int MODE, bool kSmall>
+__global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j_begin) {
+  __shared__ int64_t lds[(1 << S1) * (kBlock / ((1 << S1) / 16))];
+  const int64_t j = j_begin + blockIdx.y;
+  const int64_t b = blockIdx.z / a.dnum;
+  const int64_t di = blockIdx.z % a.dnum;
+  const KsDigit& d = a.dig[di];
+  const NttRow& row = a.rows[a.level + j];
+  if constexpr (kSmall) {
+    if constexpr (MODE == 1) {
+      modup_columns<S1>(F64Fast(row), a, d, di, b, j, lds);
+    } else {
+      modup_columns<S1>(F64Exact(row), a, d, di, b, j, lds);
+    }
+  } else {
+    modup_columns<S1>(U64Path<false>(row), a, d, di, b, j, lds);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. Forward NTT tile pass + key multiply-accumulate over every digit.
+//    Linear grid of (tile group, row, batch) work items, XCD-remapped.
+// ---------------------------------------------------------------------------
+
+// Binary64 rows: acc += w*y reduced to (-0.55q, 0.55q), with the key's
+// Montgomery value w = k R mod q (< 2q < 2^43, exact) used as a plain integer,
+// so the sum is (sum k y) R mod q -- the inverse NTT removes that R
+// (kOutStdMac).  |acc| stays below 6q over at most 10 digits.
+__device__ __forceinline__ void mac_f64(double& acc, double y, int64_t key, double q, double qinv) {
+  acc += f64_mulmod_signed(static_cast<double>(key), y, q, qinv);
+}
+
+template <class P>
+__device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g, int64_t j,
+                                          int64_t b, int64_t* lds) {
+  using V = typename P::V;
+  constexpr bool kF64 = std::is_same<V, double>::value;
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t grow = a.level + j;
+  const NttRow& row = a.rows[grow];
+  const int64_t* tw_row = a.fwd_w + grow * n;
+  const int64_t* twp_row = a.fwd_wp + grow * n;
+  const int m = threadIdx.x / 16;
+  const int t = threadIdx.x % 16;
+  const int r = static_cast<int>(g) * kTilesPerBlock + m;
+  int64_t* sh = lds + m * kTilePitch;
+  const int s0 = a.log_n - kTileLog;
+  Global<int64_t>* key = global(a.key_bases ? reinterpret_cast<const int64_t*>(a.key_bases[b]) : a.key);
+  const int64_t tile_off = static_cast<int64_t>(r) * kTile;
+  const Mont mont = load_mont(a.params, grow);
+  double facc0[16], facc1[16];
+  uint64_t uacc0[16], uacc1[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    facc0[k] = facc1[k] = 0.0;
+    uacc0[k] = uacc1[k] = 0;
+  }
+  for (int64_t di = 0; di < a.dnum; ++di) {
+    const KsDigit& d = a.dig[di];
+    const int64_t* tile = a.inter + ((b * a.dnum + di) * a.lqp + j) * n + tile_off;
+    Global<int64_t>* k0 = key + d.key_digit * a.key_sd + grow * a.key_sr + tile_off;
+    Global<int64_t>* k1 = k0 + a.key_sc;
+    // The twiddles do not depend on the digit; re-read them (L1/L2 hits)
+    // every iteration instead of keeping ~30 of them live in registers
+    // across the loop, which would halve the occupancy.
+    const int64_t* tw_opaque = tw_row;
+    const int64_t* twp_opaque = twp_row;
+    asm volatile("" : "+v"(tw_opaque), "+v"(twp_opaque));
+    const int64_t* tw = (const int64_t*)global(tw_opaque);
+    const int64_t* twp = (const int64_t*)global(twp_opaque);
+    V v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = p.unmid(tile[t + 16 * k]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int half = 8 >> s;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half], p.tw(tw, twp, (1 << (s0 + s)) + (r << s) + (k >> (4 - s))));
+      }
+    }
+    // Each thread writes exactly the words it read last time round (the
+    // final transposed read below), so no barrier is needed before this.
+#pragma unroll
+    for (int k = 0; k < 16; ++k) sh[tile_pad(t + 16 * k)] = p.mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = p.unmid(sh[17 * t + k]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int half = 8 >> s;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half],
+              p.tw(tw, twp, (1 << (s0 + 4 + s)) + (r << (4 + s)) + (t << s) + (k >> (4 - s))));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) sh[17 * t + k] = p.mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const V y = p.unmid(sh[tile_pad(t + 16 * k)]);
+      const int64_t w0 = k0[t + 16 * k];
+      const int64_t w1 = k1[t + 16 * k];
+      if constexpr (kF64) {
+        mac_f64(facc0[k], y, w0, row.qd, row.qinv);
+        mac_f64(facc1[k], y, w1, row.qd, row.qinv);
+      } else {
+        const int64_t yi = static_cast<int64_t>(y);
+        uacc0[k] = static_cast<uint64_t>(
+            add_lazy(static_cast<int64_t>(uacc0[k]), mont_mul(yi, w0, mont), mont.twice_q));
+        uacc1[k] = static_cast<uint64_t>(
+            add_lazy(static_cast<int64_t>(uacc1[k]), mont_mul(yi, w1, mont), mont.twice_q));
+      }
+    }
+    // The next digit's first LDS write targets this thread's own words
+    // (tile_pad(t + 16k)), which only this thread reads above.
+  }
+  int64_t* out0 = a.acc + ((0 * a.batch + b) * a.lqp + j) * n + tile_off;
+  int64_t* out1 = a.acc + ((1 * a.batch + b) * a.lqp + j) * n + tile_off;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if constexpr (kF64) {
+      out0[t + 16 * k] = __double_as_longlong(facc0[k]);
+      out1[t + 16 * k] = __double_as_longlong(facc1[k]);
+    } else {
+      out0[t + 16 * k] = static_cast<int64_t>(uacc0[k]);
+      out1[t + 16 * k] = static_cast<int64_t>(uacc1[k]);
+    }
+  }
+}
+
+// Block i runs on XCD i % 8; give each XCD one contiguous range of work
+// items (batch fastest) so consecutive items on an XCD share key tiles in
+// its L2.  The map is a bijection of [0, total).
+__device__ __forceinline__ int64_t xcd_remap(int64_t i, int64_t total) {
+  constexpr int64_t kXcds = 8;
+  const int64_t x = i % kXcds;
+  const int64_t s = i / kXcds;
+  const int64_t q = total / kXcds;
+  const int64_t rem = total % kXcds;
+  return x * q + (x < rem ? x : rem) + s;
+}
+
+// One launch per arithmetic path (binary64 rows, then the integer rows) so
+// each kernel is register-allocated for its own path only.
+template <int MODE, bool kSmall>
+__global__ __launch_bounds__(kBlock) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
+                                                         int64_t j_count) {
+  __shared__ int64_t lds[kTilesPerBlock * kTilePitch];
+  const int64_t total = groups * j_count * a.batch;
+  const int64_t w = xcd_remap(blockIdx.x, total);
+  const int64_t b = w % a.batch;
+  const int64_t j = j_begin + (w / a.batch) % j_count;
+  const int64_t g = w / (a.batch * j_count);
+  const NttRow& row = a.rows[a.level + j];
+  if constexpr (kSmall) {
+    if constexpr (MODE == 1) {
+      tiles_mac(F64Fast(row), a, g, j, b, lds);
+    } else {
+      tiles_mac(F64Exact(row), a, g, j, b, lds);
+    }
+  } else {
+    tiles_mac(U64Path<false>(row), a, g, j, b, lds);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. ModDown + final canonical addition.
+//    Grid (N/256, ceil(Lq / 4), 2 * B); each thread owns one coefficient of
+//    four Q rows of one accumulator.  After the inverse NTT the accumulator
+//    rows hold canonical standard residues of X over Q_l P.
+// ---------------------------------------------------------------------------
+struct ModDownArgs {
+  const int64_t* acc;       // [2, B, Lqp, N]
+  int64_t batch, lq, lqp, level;
+  int log_n;
+  ParamView params;
+  KsDigit pdig;             // the P primes as one digit (extension over QP0 columns)
+  const int64_t* p_inverse; // [Q0] P^{-1} mod q, standard
+  const NttRow* rows;       // global QP NTT records
+  View3 base0, base1, out0, out1;
+  int has_base0, has_base1;
+  const int64_t* gal_idx;   // Galois gather of base0 (rotations) or null
+  const int64_t* gal_sign;
+};
+
+__global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t c = blockIdx.z / a.batch;
+  const int64_t b = blockIdx.z % a.batch;
+  const int p_count = static_cast<int>(a.lqp - a.lq);
+  const int64_t* x = a.acc + (c * a.batch + b) * a.lqp * n + i;
+  // Mixed-radix digits of X mod P (X_P = sum_r c_r prod_{t<r} p_t).
+  int64_t dg[kMaxDigitRows];
+#pragma unroll
+  for (int r = 0; r < kMaxDigitRows; ++r) dg[r] = r < p_count ? x[(a.lq + r) * n] : 0;
+  garner(a.pdig, dg);
+  // Each digit split as c_hi 2^30 + c_lo, both exact in binary64.
+  double dlo[kMaxP], dhi[kMaxP];
+#pragma unroll
+  for (int r = 0; r < kMaxP; ++r) {
+    dlo[r] = static_cast<double>(static_cast<uint32_t>(dg[r] & ((int64_t{1} << 30) - 1)));
+    dhi[r] = static_cast<double>(static_cast<uint32_t>(dg[r] >> 30));
+  }
+  const View3& base = c == 0 ? a.base0 : a.base1;
+  const int has_base = c == 0 ? a.has_base0 : a.has_base1;
+  const int64_t* gal_idx = c == 0 ? a.gal_idx : nullptr;
+  const View3& out = c == 0 ? a.out0 : a.out1;
+  const int64_t q0 = a.pdig.moddown_stride;
+  const int64_t row_begin = static_cast<int64_t>(blockIdx.y) * kModDownRowsPerThread;
+  for (int rr = 0; rr < kModDownRowsPerThread; ++rr) {
+    const int64_t row = row_begin + rr;
+    if (row >= a.lq) break;
+    const int64_t grow = a.level + row;
+    const NttRow& nr = a.rows[grow];
+    int64_t value;
+    if (nr.small) {
+      // (x - X_P) P^{-1} mod q with X_P mod q = sum_r c_lo M_r + c_hi 2^30 M_r,
+      // all in signed binary64 (|sum| < 4.2q), canonicalised once.
+      const double q = nr.qd, qinv = nr.qinv;
+      Global<double>* tab = global(a.pdig.moddown_f64) + grow;
+      double xp = 0.0;
+      for (int r = 0; r < p_count; ++r) {
+        xp += f64_mulmod_signed(tab[r * q0], dlo[r], q, qinv);
+        xp += f64_mulmod_signed(tab[(p_count + r) * q0], dhi[r], q, qinv);
+      }
+      const double diff = u64_to_f64(static_cast<uint64_t>(x[row * n])) - xp;
+      double res = f64_mulmod_signed(tab[2 * p_count * q0], diff, q, qinv);
+      res = res < 0.0 ? res + q : res;
+      value = static_cast<int64_t>(f64_to_u64(res));
+    } else {
+      const Mont m = load_mont(a.params, grow);
+      // (X_P R) mod q by the reference basis-extension formula, then
+      // ((x R) - (X_P R)) * P^{-1} with REDC = (x - X_P) P^{-1} mod q.
+      int64_t xp = mont_mul(dg[0], m.r2, m);
+      for (int r = 1; r < p_count; ++r)
+        xp = add_lazy(xp, mont_mul(dg[r], global(a.pdig.basis)[(r - 1) * a.pdig.basis_stride + grow], m),
+                      m.twice_q);
+      const int64_t xr = mont_mul(x[row * n], m.r2, m);
+      value = mont_mul(sub_lazy(xr, xp, m.twice_q), a.p_inverse[grow], m);
+    }
+    if (has_base) {
+      const int64_t twice_q = static_cast<int64_t>(nr.twice_q);
+      const int64_t bv = read_maybe_galois(base, gal_idx, a.gal_sign, b, row, i, twice_q);
+      int64_t sum = bv + value;
+      if (sum >= twice_q) sum -= twice_q;
+      value = canonicalize(sum, twice_q);
+    }
+    out.at(b, row, i) = value;
+  }
+}
+
+}  // namespace
+}  // namespace fh
+
+using namespace fh;
+
+// One fused key switch of a batched component.  See include/fhelium_amd.h.
+FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp,
+                              int64_t level, int64_t log_n, int64_t small_rows,
+                              const int64_t* digit_table,
+                              int64_t digit_count, const int64_t* p_digit_table,
+                              const int64_t* p_inverse_q, fh_table2 rns_params, const void* ntt_rows,
+                              const int64_t* forward_twiddles, const int64_t* forward_twiddles_shoup,
+                              const int64_t* inverse_twiddles, const int64_t* inverse_twiddles_shoup,
+                              const int64_t* key, const int64_t* key_bases, int64_t key_stride_digit,
+                              int64_t key_stride_comp, int64_t key_stride_row,
+                              const int64_t* galois_index_tables, const int64_t* galois_sign_tables,
+                              int64_t* workspace_digits, int64_t* workspace_inter,
+                              int64_t* workspace_acc, fh_view3 base0, fh_view3 base1, fh_view3 out0,
+                              fh_view3 out1, int32_t mode, void* stream) {
+  FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "fused key switch supports logN 12..16");
+  FH_CHECK_ARG(batch >= 1 && batch <= 32767, "fused key switch batch %lld out of range",
+               (long long)batch);
+  FH_CHECK_ARG(lq >= 1 && lqp > lq && lqp - lq <= kMaxP, "bad Q/QP row counts");
+  FH_CHECK_ARG(digit_count >= 1 && digit_count * batch <= 65535, "bad digit count");
+  FH_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (exact) or 1 (fast)");
+  FH_CHECK_ARG(component.data && digit_table && p_digit_table && p_inverse_q && rns_params.data &&
+                   ntt_rows && forward_twiddles && forward_twiddles_shoup && inverse_twiddles &&
+                   inverse_twiddles_shoup && workspace_digits && workspace_inter && workspace_acc &&
+                   out0.data && out1.data,
+               "fused key switch: null operand");
+  FH_CHECK_ARG((key != nullptr) != (key_bases != nullptr), "pass exactly one of key / key_bases");
+  FH_CHECK_ARG((galois_index_tables == nullptr) == (galois_sign_tables == nullptr),
+               "Galois index and sign tables go together");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t n = int64_t{1} << log_n;
+  KsArgs a;
+  a.src = View3(component);
+  a.digits = workspace_digits;
+  a.inter = workspace_inter;
+  a.acc = workspace_acc;
+  a.dig = reinterpret_cast<const KsDigit*>(digit_table);
+  a.dnum = digit_count;
+  a.batch = batch;
+  a.lq = lq;
+  a.lqp = lqp;
+  a.level = level;
+  a.log_n = static_cast<int>(log_n);
+  a.mode = mode;
+  a.params = ParamView{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  a.rows = static_cast<const NttRow*>(ntt_rows);
+  a.fwd_w = forward_twiddles;
+  a.fwd_wp = forward_twiddles_shoup;
+  a.key = key;
+  a.key_bases = key_bases;
+  a.key_sd = key_stride_digit;
+  a.key_sc = key_stride_comp;
+  a.key_sr = key_stride_row;
+  a.gal_idx = galois_index_tables;
+  a.gal_sign = galois_sign_tables;
+
+  hipLaunchKernelGGL(k_ks_decompose,
+                     dim3(static_cast<uint32_t>(ceil_div(n, kBlock)), static_cast<uint32_t>(digit_count),
+                          static_cast<uint32_t>(batch)),
+                     dim3(kBlock), 0, s, a);
+  FH_CHECK_LAUNCH("key switch decompose");
+
+  // Rows below 2^46 take the binary64 path (the same rule as the NTT row
+  // records); they precede the integer rows in the QP chain.
+  FH_CHECK_ARG(small_rows >= 0 && small_rows <= lqp, "bad binary64 row count");
+  const int s1 = static_cast<int>(log_n) - kTileLog;
+  const int T = (1 << s1) / 16;
+  const int C = kBlock / T;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int64_t j_begin = pass == 0 ? 0 : small_rows;
+    const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
+    if (j_count == 0) continue;
+    const dim3 gcol(kTile / C, static_cast<uint32_t>(j_count), static_cast<uint32_t>(digit_count * batch));
+#define FH_MODUP_CASE(S)                                                                          \
+  case S:                                                                                         \
+    if (pass == 1) {                                                                              \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, false>), gcol, dim3(kBlock), 0, s, a, j_begin); \
+    } else if (mode == 1) {                                                                       \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 1, true>), gcol, dim3(kBlock), 0, s, a, j_begin);  \
+    } else {                                                                                      \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, true>), gcol, dim3(kBlock), 0, s, a, j_begin);  \
+    }                                                                                             \
+    break;
+    switch (s1) {
+      FH_MODUP_CASE(4)
+      FH_MODUP_CASE(5)
+      FH_MODUP_CASE(6)
+      FH_MODUP_CASE(7)
+      default:
+        FH_MODUP_CASE(8)
+    }
+#undef FH_MODUP_CASE
+  }
+  FH_CHECK_LAUNCH("key switch modup + column pass");
+
+  const int64_t groups = n / (kTile * kTilesPerBlock);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int64_t j_begin = pass == 0 ? 0 : small_rows;
+    const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
+    if (j_count == 0) continue;
+    const int64_t total = groups * j_count * batch;
+    FH_CHECK_ARG(total <= 0x7fffffff, "key switch grid too large");
+    const dim3 grid(static_cast<uint32_t>(total));
+    if (pass == 1) {
+      hipLaunchKernelGGL((k_ks_tiles_mac<0, false>), grid, dim3(kBlock), 0, s, a, groups, j_begin, j_count);
+    } else if (mode == 1) {
+      hipLaunchKernelGGL((k_ks_tiles_mac<1, true>), grid, dim3(kBlock), 0, s, a, groups, j_begin, j_count);
+    } else {
+      hipLaunchKernelGGL((k_ks_tiles_mac<0, true>), grid, dim3(kBlock), 0, s, a, groups, j_begin, j_count);
+    }
+  }
+  FH_CHECK_LAUNCH("key switch tile pass + MAC");
+
+  // Both accumulators as one [2B, Lqp, N] inverse transform.
+  int st = ntt::inverse_internal(workspace_acc, 2 * batch, lqp, static_cast<int>(log_n), lqp * n, n,
+                                 static_cast<const NttRow*>(ntt_rows), inverse_twiddles,
+                                 inverse_twiddles_shoup, level, kOutStdMac, true, s);
+  if (st != kOk) return st;
+
+  ModDownArgs md;
+  md.acc = workspace_acc;
+  md.batch = batch;
+  md.lq = lq;
+  md.lqp = lqp;
+  md.level = level;
+  md.log_n = static_cast<int>(log_n);
+  md.params = a.params;
+  md.rows = static_cast<const NttRow*>(ntt_rows);
+  md.pdig = *reinterpret_cast<const KsDigit*>(p_digit_table);
+  md.p_inverse = p_inverse_q;
+  md.base0 = View3(base0);
+  md.base1 = View3(base1);
+  md.out0 = View3(out0);
+  md.out1 = View3(out1);
+  md.has_base0 = base0.data != nullptr ? 1 : 0;
+  md.has_base1 = base1.data != nullptr ? 1 : 0;
+  md.gal_idx = galois_index_tables;
+  md.gal_sign = galois_sign_tables;
+  hipLaunchKernelGGL(k_ks_moddown_add,
+                     dim3(static_cast<uint32_t>(ceil_div(n, kBlock)),
+                          static_cast<uint32_t>(ceil_div(lq, kModDownRowsPerThread)),
+                          static_cast<uint32_t>(2 * batch)),
+                     dim3(kBlock), 0, s, md);
+  FH_CHECK_LAUNCH("key switch moddown + add");
+  return kOk;
+}

@@ -35,190 +35,12 @@
 // one launch per stage.
 #include "fh_common.h"
 
+#include "ntt_core.h"
+
 namespace fh {
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kTileLog = 8;
-constexpr int kTile = 1 << kTileLog;      // contiguous tile of the tile pass
-constexpr int kTilesPerBlock = 16;
-constexpr int kTilePitch = kTile + kTile / 16;  // one pad word per 16
-
-__device__ __forceinline__ int tile_pad(int x) { return x + (x >> 4); }
-
-// Inverse output representations (fh_ntt_inverse).
-enum InvOut : int { kOutMontgomery = 0, kOutStdLazy = 1, kOutStd = 2, kOutCentered = 3 };
-
-__device__ __forceinline__ int64_t center_canonical(int64_t r, uint64_t q) {
-  const int64_t qq = static_cast<int64_t>(q);
-  return r <= (qq >> 1) ? r : r - qq;
-}
-
-// ---------------------------------------------------------------------------
-// Arithmetic paths.  Interface:
-//   in(raw)          input residue (lazy [0, 2q)) -> working value
-//   mid/unmid        working value <-> 64-bit intermediate between passes
-//   out_fwd(v)       final forward value -> int64 residue
-//   out_inv(v, f)    N^{-1} scaling and representation f -> int64 residue
-//   fwd/inv          butterflies; mul_const for to-Montgomery scaling.
-// ---------------------------------------------------------------------------
-
-// Exact binary64 path (q < 2^46): reference lazy semantics, see fh_common.h.
-struct F64Exact {
-  using V = double;
-  struct Tw {
-    double w;
-  };
-  double q, twoq, qinv;
-  const NttRow* row;
-  __device__ __forceinline__ explicit F64Exact(const NttRow& r)
-      : q(r.qd), twoq(2.0 * r.qd), qinv(r.qinv), row(&r) {}
-  __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t*, int64_t idx) const {
-    return Tw{__longlong_as_double(w[idx])};
-  }
-  __device__ __forceinline__ Tw konst(uint64_t c, uint64_t) const {
-    return Tw{static_cast<double>(c)};
-  }
-  __device__ __forceinline__ V in(int64_t raw) const { return u64_to_f64(static_cast<uint64_t>(raw)); }
-  __device__ __forceinline__ int64_t mid(V v) const { return __double_as_longlong(v); }
-  __device__ __forceinline__ V unmid(int64_t b) const { return __longlong_as_double(b); }
-  __device__ __forceinline__ V mul(Tw w, V x) const { return f64_mulmod(w.w, x, q, qinv); }
-  __device__ __forceinline__ V add(V a, V b) const {
-    const V s = a + b;
-    return s >= twoq ? s - twoq : s;
-  }
-  __device__ __forceinline__ V sub(V a, V b) const {
-    const V d = (a + twoq) - b;
-    return d >= twoq ? d - twoq : d;
-  }
-  __device__ __forceinline__ void fwd(V& u, V& l, Tw w) const {
-    const V v = mul(w, l);
-    const V e = add(u, v);
-    l = sub(u, v);
-    u = e;
-  }
-  __device__ __forceinline__ void inv(V& u, V& l, Tw w) const {
-    const V d = sub(u, l);
-    u = add(u, l);
-    l = mul(w, d);
-  }
-  __device__ __forceinline__ V to_mont(V x) const { return mul(konst(row->to_mont, 0), x); }
-  __device__ __forceinline__ int64_t out_fwd(V v) const {
-    return static_cast<int64_t>(f64_to_u64(v));
-  }
-  __device__ __forceinline__ int64_t out_inv(V v, int form) const {
-    const Tw c = form == kOutMontgomery ? konst(row->ninv_mont, 0) : konst(row->ninv_std, 0);
-    int64_t r = static_cast<int64_t>(f64_to_u64(mul(c, v)));
-    return form == kOutCentered ? center_canonical(r, row->q) : r;
-  }
-};
-
-// Fast binary64 path (q < 2^46): signed values, products reduced to
-// (-0.52q, 0.52q) without the final correction.
-struct F64Fast {
-  using V = double;
-  using Tw = F64Exact::Tw;
-  double q, qinv;
-  const NttRow* row;
-  __device__ __forceinline__ explicit F64Fast(const NttRow& r) : q(r.qd), qinv(r.qinv), row(&r) {}
-  __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t*, int64_t idx) const {
-    return Tw{__longlong_as_double(w[idx])};
-  }
-  __device__ __forceinline__ Tw konst(uint64_t c, uint64_t) const {
-    return Tw{static_cast<double>(c)};
-  }
-  __device__ __forceinline__ V in(int64_t raw) const { return u64_to_f64(static_cast<uint64_t>(raw)); }
-  __device__ __forceinline__ int64_t mid(V v) const { return __double_as_longlong(v); }
-  __device__ __forceinline__ V unmid(int64_t b) const { return __longlong_as_double(b); }
-  // w*x - round(w*x/q) q, exact; |result| < 0.52 q for |x| < 2^45.
-  __device__ __forceinline__ V mul(Tw w, V x) const {
-    const double h = w.w * x;
-    const double l = __builtin_fma(w.w, x, -h);
-    const double t = __builtin_fma(h, qinv, kRoundMagic) - kRoundMagic;
-    return __builtin_fma(-t, q, h) + l;
-  }
-  // x - round(x/q) q, exact for |x| < 2^51; |result| <= q/2 (+ulp-level).
-  __device__ __forceinline__ V reduce(V x) const {
-    const double t = __builtin_fma(x, qinv, kRoundMagic) - kRoundMagic;
-    return __builtin_fma(-t, q, x);
-  }
-  __device__ __forceinline__ V canon(V x) const {
-    const V r = reduce(x);
-    return r < 0.0 ? r + q : r;
-  }
-  __device__ __forceinline__ void fwd(V& u, V& l, Tw w) const {
-    const V v = mul(w, l);
-    l = u - v;
-    u = u + v;
-  }
-  __device__ __forceinline__ void inv(V& u, V& l, Tw w) const {
-    const V d = u - l;
-    u = reduce(u + l);
-    l = mul(w, d);
-  }
-  __device__ __forceinline__ V to_mont(V x) const { return mul(konst(row->to_mont, 0), x); }
-  __device__ __forceinline__ int64_t out_fwd(V v) const {
-    return static_cast<int64_t>(f64_to_u64(canon(v)));
-  }
-  __device__ __forceinline__ int64_t out_inv(V v, int form) const {
-    const Tw c = form == kOutMontgomery ? konst(row->ninv_mont, 0) : konst(row->ninv_std, 0);
-    int64_t r = static_cast<int64_t>(f64_to_u64(canon(mul(c, v))));
-    return form == kOutCentered ? center_canonical(r, row->q) : r;
-  }
-};
-
-// Integer Shoup path (any q < 2^62), reference lazy semantics.  With
-// kCanonicalOut the forward output is reduced to [0, q).
-template <bool kCanonicalOut>
-struct U64Path {
-  using V = uint64_t;
-  struct Tw {
-    uint64_t w, wp;
-  };
-  uint64_t q, twoq;
-  const NttRow* row;
-  __device__ __forceinline__ explicit U64Path(const NttRow& r) : q(r.q), twoq(r.twice_q), row(&r) {}
-  __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t* wp, int64_t idx) const {
-    return Tw{static_cast<uint64_t>(w[idx]), static_cast<uint64_t>(wp[idx])};
-  }
-  __device__ __forceinline__ Tw konst(uint64_t c, uint64_t cp) const { return Tw{c, cp}; }
-  __device__ __forceinline__ V in(int64_t raw) const { return static_cast<uint64_t>(raw); }
-  __device__ __forceinline__ int64_t mid(V v) const { return static_cast<int64_t>(v); }
-  __device__ __forceinline__ V unmid(int64_t b) const { return static_cast<uint64_t>(b); }
-  __device__ __forceinline__ V mul(Tw w, V x) const { return shoup_mulmod(w.w, w.wp, x, q); }
-  __device__ __forceinline__ V add(V a, V b) const {
-    const V s = a + b;
-    return s >= twoq ? s - twoq : s;
-  }
-  __device__ __forceinline__ V sub(V a, V b) const {
-    const V d = (a + twoq) - b;
-    return d >= twoq ? d - twoq : d;
-  }
-  __device__ __forceinline__ void fwd(V& u, V& l, Tw w) const {
-    const V v = mul(w, l);
-    const V e = add(u, v);
-    l = sub(u, v);
-    u = e;
-  }
-  __device__ __forceinline__ void inv(V& u, V& l, Tw w) const {
-    const V d = sub(u, l);
-    u = add(u, l);
-    l = mul(w, d);
-  }
-  __device__ __forceinline__ V to_mont(V x) const {
-    return mul(konst(row->to_mont, row->to_mont_p), x);
-  }
-  __device__ __forceinline__ int64_t out_fwd(V v) const {
-    if (kCanonicalOut) v = v >= q ? v - q : v;
-    return static_cast<int64_t>(v);
-  }
-  __device__ __forceinline__ int64_t out_inv(V v, int form) const {
-    const Tw c = form == kOutMontgomery ? konst(row->ninv_mont, row->ninv_mont_p)
-                                        : konst(row->ninv_std, row->ninv_std_p);
-    int64_t r = static_cast<int64_t>(mul(c, v));
-    return form == kOutCentered ? center_canonical(r, q) : r;
-  }
-};
+using namespace ::fh::ntt;
 
 // ---------------------------------------------------------------------------
 // Pass bodies.  `base` is the operand row (int64 residues or intermediates).
@@ -330,7 +152,7 @@ __device__ __forceinline__ void fwd_tiles(const P& p, int log_n, int64_t* base, 
 // residues, writes intermediates.
 template <class P>
 __device__ __forceinline__ void inv_tiles(const P& p, int log_n, int64_t* base, const int64_t* tw,
-                                          const int64_t* twp, int64_t* lds) {
+                                          const int64_t* twp, bool in_mid, int64_t* lds) {
   using V = typename P::V;
   const int m = threadIdx.x / 16;
   const int t = threadIdx.x % 16;
@@ -343,7 +165,7 @@ __device__ __forceinline__ void inv_tiles(const P& p, int log_n, int64_t* base, 
   __syncthreads();
   V v[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = p.in(sh[17 * t + k]);
+  for (int k = 0; k < 16; ++k) v[k] = in_mid ? p.unmid(sh[17 * t + k]) : p.in(sh[17 * t + k]);
   // Round A: stages 0..3 pair tile bits 0..3 (bits of k).
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -441,6 +263,7 @@ struct NttArgs {
   int64_t row_start;    // parameter row of limb 0
   int log_n;
   int form;             // forward: to-Montgomery flag; inverse: InvOut
+  int in_mid;           // inverse only: input already in the working format
 };
 
 struct RowRef {
@@ -501,7 +324,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_inv_tiles(NttArgs a) {
   __shared__ int64_t lds[kTilesPerBlock * kTilePitch];
   const RowRef rr = row_ref(a);
-  FH_DISPATCH_PATH(MODE, *rr.row, (inv_tiles(p, a.log_n, rr.base, rr.tw, rr.twp, lds)));
+  FH_DISPATCH_PATH(MODE, *rr.row, (inv_tiles(p, a.log_n, rr.base, rr.tw, rr.twp, a.in_mid != 0, lds)));
 }
 
 template <int S1, int MODE>
@@ -737,6 +560,8 @@ __global__ void k_prepare_rows(ParamView params, int64_t limbs, NttRow* rows) {
   r.to_mont = static_cast<uint64_t>(mont_mul(m.r2, 1, m));
   r.ninv_mont = static_cast<uint64_t>(mont_mul(m.ninv, 1, m));
   r.ninv_std = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(r.ninv_mont), 1, m));
+  r.ninv_std_rinv = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(r.ninv_std), 1, m));
+  r.ninv_std_rinv_p = shoup_companion(r.ninv_std_rinv, m.q);
   r.to_mont_p = shoup_companion(r.to_mont, m.q);
   r.ninv_mont_p = shoup_companion(r.ninv_mont, m.q);
   r.ninv_std_p = shoup_companion(r.ninv_std, m.q);
@@ -770,6 +595,18 @@ int check_ntt_common(int64_t batch, int64_t limbs, int64_t log_n, const void* ro
 }
 
 }  // namespace
+
+namespace ntt {
+
+int inverse_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n, int64_t batch_stride,
+                     int64_t limb_stride, const NttRow* rows, const int64_t* tw, const int64_t* twp,
+                     int64_t row_start, int output_form, bool in_mid, hipStream_t stream) {
+  NttArgs a{data, batch_stride, limb_stride, limbs, 0, rows, tw, twp, row_start, log_n, output_form,
+            in_mid ? 1 : 0};
+  return launch(a, batch * limbs, false, kFast, stream);
+}
+
+}  // namespace ntt
 }  // namespace fh
 
 using namespace fh;
@@ -783,7 +620,7 @@ FH_API int fh_ntt_forward(int64_t* residues, int64_t batch, int64_t limbs, int64
   if (batch == 0 || limbs == 0) return kOk;
   FH_CHECK_ARG(residues != nullptr, "NTT residues must be non-null");
   NttArgs a{residues, batch_stride, limb_stride, limbs, 0, static_cast<const NttRow*>(ntt_rows),
-            twiddles, twiddles_shoup, row_start, static_cast<int>(log_n), to_montgomery ? 1 : 0};
+            twiddles, twiddles_shoup, row_start, static_cast<int>(log_n), to_montgomery ? 1 : 0, 0};
   return launch(a, batch * limbs, true, mode, static_cast<hipStream_t>(stream));
 }
 
@@ -798,7 +635,7 @@ FH_API int fh_ntt_inverse(int64_t* residues, int64_t batch, int64_t limbs, int64
   if (batch == 0 || limbs == 0) return kOk;
   FH_CHECK_ARG(residues != nullptr, "NTT residues must be non-null");
   NttArgs a{residues, batch_stride, limb_stride, limbs, 0, static_cast<const NttRow*>(ntt_rows),
-            twiddles, twiddles_shoup, row_start, static_cast<int>(log_n), output_form};
+            twiddles, twiddles_shoup, row_start, static_cast<int>(log_n), output_form, 0};
   return launch(a, batch * limbs, false, mode, static_cast<hipStream_t>(stream));
 }
 

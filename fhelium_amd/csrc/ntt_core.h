@@ -1,0 +1,203 @@
+// Shared NTT arithmetic for the gfx950 kernels (ntt.hip, keyswitch.hip):
+// launch constants and the three butterfly arithmetic paths.  See ntt.hip
+// for the transform semantics and fh_common.h for the exactness argument of
+// the binary64 and Shoup modular products.
+#pragma once
+
+#include "fh_common.h"
+
+namespace fh {
+namespace ntt {
+constexpr int kBlock = 256;
+constexpr int kTileLog = 8;
+constexpr int kTile = 1 << kTileLog;      // contiguous tile of the tile pass
+constexpr int kTilesPerBlock = 16;
+constexpr int kTilePitch = kTile + kTile / 16;  // one pad word per 16
+
+__device__ __forceinline__ int tile_pad(int x) { return x + (x >> 4); }
+
+// Inverse output representations (fh_ntt_inverse).
+// kOutStdMac: standard canonical output of accumulators whose binary64 rows
+// hold x*R mod q (a key MAC that skipped REDC's R^{-1}); integer rows are
+// ordinary Montgomery values and get the plain standard output.
+enum InvOut : int { kOutMontgomery = 0, kOutStdLazy = 1, kOutStd = 2, kOutCentered = 3, kOutStdMac = 4 };
+
+__device__ __forceinline__ int64_t center_canonical(int64_t r, uint64_t q) {
+  const int64_t qq = static_cast<int64_t>(q);
+  return r <= (qq >> 1) ? r : r - qq;
+}
+
+// ---------------------------------------------------------------------------
+// Arithmetic paths.  Interface:
+//   in(raw)          input residue (lazy [0, 2q)) -> working value
+//   mid/unmid        working value <-> 64-bit intermediate between passes
+//   out_fwd(v)       final forward value -> int64 residue
+//   out_inv(v, f)    N^{-1} scaling and representation f -> int64 residue
+//   fwd/inv          butterflies; mul_const for to-Montgomery scaling.
+// ---------------------------------------------------------------------------
+
+// Exact binary64 path (q < 2^46): reference lazy semantics, see fh_common.h.
+struct F64Exact {
+  using V = double;
+  struct Tw {
+    double w;
+  };
+  double q, twoq, qinv;
+  const NttRow* row;
+  __device__ __forceinline__ explicit F64Exact(const NttRow& r)
+      : q(r.qd), twoq(2.0 * r.qd), qinv(r.qinv), row(&r) {}
+  __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t*, int64_t idx) const {
+    return Tw{__longlong_as_double(w[idx])};
+  }
+  __device__ __forceinline__ Tw konst(uint64_t c, uint64_t) const {
+    return Tw{static_cast<double>(c)};
+  }
+  __device__ __forceinline__ V in(int64_t raw) const { return u64_to_f64(static_cast<uint64_t>(raw)); }
+  __device__ __forceinline__ int64_t mid(V v) const { return __double_as_longlong(v); }
+  __device__ __forceinline__ V unmid(int64_t b) const { return __longlong_as_double(b); }
+  __device__ __forceinline__ V mul(Tw w, V x) const { return f64_mulmod(w.w, x, q, qinv); }
+  __device__ __forceinline__ V add(V a, V b) const {
+    const V s = a + b;
+    return s >= twoq ? s - twoq : s;
+  }
+  __device__ __forceinline__ V sub(V a, V b) const {
+    const V d = (a + twoq) - b;
+    return d >= twoq ? d - twoq : d;
+  }
+  __device__ __forceinline__ void fwd(V& u, V& l, Tw w) const {
+    const V v = mul(w, l);
+    const V e = add(u, v);
+    l = sub(u, v);
+    u = e;
+  }
+  __device__ __forceinline__ void inv(V& u, V& l, Tw w) const {
+    const V d = sub(u, l);
+    u = add(u, l);
+    l = mul(w, d);
+  }
+  __device__ __forceinline__ V to_mont(V x) const { return mul(konst(row->to_mont, 0), x); }
+  __device__ __forceinline__ int64_t out_fwd(V v) const {
+    return static_cast<int64_t>(f64_to_u64(v));
+  }
+  __device__ __forceinline__ int64_t out_inv(V v, int form) const {
+    const Tw c = form == kOutMontgomery ? konst(row->ninv_mont, 0)
+                 : form == kOutStdMac      ? konst(row->ninv_std_rinv, 0)
+                                           : konst(row->ninv_std, 0);
+    int64_t r = static_cast<int64_t>(f64_to_u64(mul(c, v)));
+    return form == kOutCentered ? center_canonical(r, row->q) : r;
+  }
+};
+
+// Fast binary64 path (q < 2^46): signed values, products reduced to
+// (-0.52q, 0.52q) without the final correction.
+struct F64Fast {
+  using V = double;
+  using Tw = F64Exact::Tw;
+  double q, qinv;
+  const NttRow* row;
+  __device__ __forceinline__ explicit F64Fast(const NttRow& r) : q(r.qd), qinv(r.qinv), row(&r) {}
+  __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t*, int64_t idx) const {
+    return Tw{__longlong_as_double(w[idx])};
+  }
+  __device__ __forceinline__ Tw konst(uint64_t c, uint64_t) const {
+    return Tw{static_cast<double>(c)};
+  }
+  __device__ __forceinline__ V in(int64_t raw) const { return u64_to_f64(static_cast<uint64_t>(raw)); }
+  __device__ __forceinline__ int64_t mid(V v) const { return __double_as_longlong(v); }
+  __device__ __forceinline__ V unmid(int64_t b) const { return __longlong_as_double(b); }
+  // w*x - round(w*x/q) q, exact; |result| < 0.52 q for |x| < 2^45.
+  __device__ __forceinline__ V mul(Tw w, V x) const { return f64_mulmod_signed(w.w, x, q, qinv); }
+  // x - round(x/q) q, exact for |x| < 2^51; |result| <= q/2 (+ulp-level).
+  __device__ __forceinline__ V reduce(V x) const {
+    const double t = __builtin_fma(x, qinv, kRoundMagic) - kRoundMagic;
+    return __builtin_fma(-t, q, x);
+  }
+  __device__ __forceinline__ V canon(V x) const {
+    const V r = reduce(x);
+    return r < 0.0 ? r + q : r;
+  }
+  __device__ __forceinline__ void fwd(V& u, V& l, Tw w) const {
+    const V v = mul(w, l);
+    l = u - v;
+    u = u + v;
+  }
+  __device__ __forceinline__ void inv(V& u, V& l, Tw w) const {
+    const V d = u - l;
+    u = reduce(u + l);
+    l = mul(w, d);
+  }
+  __device__ __forceinline__ V to_mont(V x) const { return mul(konst(row->to_mont, 0), x); }
+  __device__ __forceinline__ int64_t out_fwd(V v) const {
+    return static_cast<int64_t>(f64_to_u64(canon(v)));
+  }
+  __device__ __forceinline__ int64_t out_inv(V v, int form) const {
+    const Tw c = form == kOutMontgomery ? konst(row->ninv_mont, 0)
+                 : form == kOutStdMac      ? konst(row->ninv_std_rinv, 0)
+                                           : konst(row->ninv_std, 0);
+    int64_t r = static_cast<int64_t>(f64_to_u64(canon(mul(c, v))));
+    return form == kOutCentered ? center_canonical(r, row->q) : r;
+  }
+};
+
+// Integer Shoup path (any q < 2^62), reference lazy semantics.  With
+// kCanonicalOut the forward output is reduced to [0, q).
+template <bool kCanonicalOut>
+struct U64Path {
+  using V = uint64_t;
+  struct Tw {
+    uint64_t w, wp;
+  };
+  uint64_t q, twoq;
+  const NttRow* row;
+  __device__ __forceinline__ explicit U64Path(const NttRow& r) : q(r.q), twoq(r.twice_q), row(&r) {}
+  __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t* wp, int64_t idx) const {
+    return Tw{static_cast<uint64_t>(w[idx]), static_cast<uint64_t>(wp[idx])};
+  }
+  __device__ __forceinline__ Tw konst(uint64_t c, uint64_t cp) const { return Tw{c, cp}; }
+  __device__ __forceinline__ V in(int64_t raw) const { return static_cast<uint64_t>(raw); }
+  __device__ __forceinline__ int64_t mid(V v) const { return static_cast<int64_t>(v); }
+  __device__ __forceinline__ V unmid(int64_t b) const { return static_cast<uint64_t>(b); }
+  __device__ __forceinline__ V mul(Tw w, V x) const { return shoup_mulmod(w.w, w.wp, x, q); }
+  __device__ __forceinline__ V add(V a, V b) const {
+    const V s = a + b;
+    return s >= twoq ? s - twoq : s;
+  }
+  __device__ __forceinline__ V sub(V a, V b) const {
+    const V d = (a + twoq) - b;
+    return d >= twoq ? d - twoq : d;
+  }
+  __device__ __forceinline__ void fwd(V& u, V& l, Tw w) const {
+    const V v = mul(w, l);
+    const V e = add(u, v);
+    l = sub(u, v);
+    u = e;
+  }
+  __device__ __forceinline__ void inv(V& u, V& l, Tw w) const {
+    const V d = sub(u, l);
+    u = add(u, l);
+    l = mul(w, d);
+  }
+  __device__ __forceinline__ V to_mont(V x) const {
+    return mul(konst(row->to_mont, row->to_mont_p), x);
+  }
+  __device__ __forceinline__ int64_t out_fwd(V v) const {
+    if (kCanonicalOut) v = v >= q ? v - q : v;
+    return static_cast<int64_t>(v);
+  }
+  __device__ __forceinline__ int64_t out_inv(V v, int form) const {
+    const Tw c = form == kOutMontgomery ? konst(row->ninv_mont, row->ninv_mont_p)
+                                        : konst(row->ninv_std, row->ninv_std_p);
+    int64_t r = static_cast<int64_t>(mul(c, v));
+    return form == kOutCentered ? center_canonical(r, q) : r;
+  }
+};
+
+// Inverse transform launcher shared with keyswitch.hip (fast arithmetic).
+// With in_mid the input rows hold working-format values (binary64 bits for
+// rows below 2^46, u64 otherwise) instead of int64 residues.
+int inverse_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n, int64_t batch_stride,
+                     int64_t limb_stride, const NttRow* rows, const int64_t* tw, const int64_t* twp,
+                     int64_t row_start, int output_form, bool in_mid, hipStream_t stream);
+
+}  // namespace ntt
+}  // namespace fh

@@ -31,12 +31,16 @@ from fhelium_amd.runtime import RnsRuntime
 class CkksEngine:
     def __init__(self, preset: Preset | str | CkksConfig = Preset.slots32768_scale40_levels34_int64,
                  *, device: str | torch.device = "cuda", ntt_mode: str = "fast",
-                 seed: int = 0x12345678, allow_secret_key_generation: bool = True) -> None:
+                 seed: int = 0x12345678, allow_secret_key_generation: bool = True,
+                 fused_keyswitch: bool = True) -> None:
         self.config = preset if isinstance(preset, CkksConfig) else CkksConfig.from_preset(preset)
         self.runtime = RnsRuntime(self.config, device, ntt_mode)
         self.device = self.runtime.device
         self.layout = self.runtime.layout
         self.key_switcher = HybridKeySwitcher(self.runtime)
+        # fused: six-launch key switch (fh_keyswitch_fused); otherwise the
+        # reference's per-digit op sequence.  Both are bit-identical.
+        self.fused_keyswitch = fused_keyswitch
         self.allow_secret_key_generation = allow_secret_key_generation
         # Seeded device sampler (Philox); key and ciphertext randomness never
         # leaves HBM.
@@ -243,10 +247,12 @@ class CkksEngine:
         rt = self.runtime
         d = ct.data.clone()
         rt.ntt.inverse_to_standard_(d, rt.row_start(ct.level))
-        corr0, corr1 = self.key_switcher.apply_key_switch(d[2], key, ct.level)
-        params = rt.params(ct.level)
-        out = torch.empty((2, *d.shape[1:]), dtype=torch.int64, device=self.device)
-        K.add_two_canonicalize(out, d[0], d[1], corr0, corr1, params)
+        if self.fused_keyswitch:
+            out = self.key_switcher.key_switch_fused(d[2], key, ct.level, base0=d[0], base1=d[1])
+        else:
+            corr0, corr1 = self.key_switcher.apply_key_switch(d[2], key, ct.level)
+            out = torch.empty((2, *d.shape[1:]), dtype=torch.int64, device=self.device)
+            K.add_two_canonicalize(out, d[0], d[1], corr0, corr1, rt.params(ct.level))
         return Ciphertext(out, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
 
     def rescale_to_next_level(self, ct: Ciphertext) -> Ciphertext:
@@ -287,13 +293,31 @@ class CkksEngine:
         self._assert_rotation_ciphertext(ct)
         rt = self.runtime
         g = key.galois_element
+        if self.fused_keyswitch:
+            entries = int(np.prod(ct.batch_shape)) if ct.batch_shape else 1
+            return self._switch_rotated_fused(ct, key, [g] * entries)
         idx, sgn = rt.coefficient_galois(g)
         twice = rt.twice_modulus[rt.row_start(ct.level): self.config.num_q_primes]
         rotated = K.apply_coefficient_galois_automorphism(ct.data, idx, sgn, twice)
-        corr0, corr1 = self.key_switcher.apply_key_switch(rotated[1], key, ct.level)
-        out = torch.empty_like(rotated)
-        out[0] = K.add_canonical(rotated[0], corr0, rt.params(ct.level))
-        out[1] = corr1
+        return self._switch_rotated(ct, rotated, key)
+
+    def _switch_rotated_fused(self, ct: Ciphertext, key, galois_elements: list[int]) -> Ciphertext:
+        """Automorphism folded into the key switch: the kernels read c1 and
+        c0 through each entry's Galois gather."""
+
+        out = self.key_switcher.key_switch_fused(ct.data[1], key, ct.level, base0=ct.data[0],
+                                                 galois_elements=galois_elements)
+        return Ciphertext(out, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
+
+    def _switch_rotated(self, ct: Ciphertext, rotated: torch.Tensor, key) -> Ciphertext:
+        rt = self.runtime
+        if self.fused_keyswitch:
+            out = self.key_switcher.key_switch_fused(rotated[1], key, ct.level, base0=rotated[0])
+        else:
+            corr0, corr1 = self.key_switcher.apply_key_switch(rotated[1], key, ct.level)
+            out = torch.empty_like(rotated)
+            out[0] = K.add_canonical(rotated[0], corr0, rt.params(ct.level))
+            out[1] = corr1
         return Ciphertext(out, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
 
     def rotate_by_steps(self, ct: Ciphertext, rotation_steps: list[int]) -> Ciphertext:
@@ -307,16 +331,14 @@ class CkksEngine:
             raise ValueError("rotate_by_steps needs a [2, B, limb, N] ciphertext and B steps")
         rt = self.runtime
         keys = [self.rotation_key(s) for s in rotation_steps]
+        if self.fused_keyswitch:
+            return self._switch_rotated_fused(ct, keys, [k.galois_element for k in keys])
         twice = rt.twice_modulus[rt.row_start(ct.level): self.config.num_q_primes]
         rotated = torch.empty_like(ct.data)
         for b, key in enumerate(keys):
             idx, sgn = rt.coefficient_galois(key.galois_element)
             rotated[:, b] = K.apply_coefficient_galois_automorphism(ct.data[:, b], idx, sgn, twice)
-        corr0, corr1 = self.key_switcher.apply_key_switch(rotated[1], keys, ct.level)
-        out = torch.empty_like(rotated)
-        out[0] = K.add_canonical(rotated[0], corr0, rt.params(ct.level))
-        out[1] = corr1
-        return Ciphertext(out, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
+        return self._switch_rotated(ct, rotated, keys)
 
     def rotate_many_with_keys(self, ct: Ciphertext, keys: list[RotationKey], *,
                               use_hoisting: bool = True) -> list[Ciphertext]:

@@ -19,11 +19,149 @@ from fhelium_amd.core import KeySwitchKey
 from fhelium_amd.runtime import RnsRuntime
 
 
+# Primes below this bound use the binary64 kernels (matches the NTT row records).
+SMALL_PRIME_LIMIT = 1 << 46
+
+
 class HybridKeySwitcher:
     def __init__(self, runtime: RnsRuntime) -> None:
         self.rt = runtime
         self.layout = runtime.layout
         self.config = runtime.config
+        self._digit_tables: dict[int, torch.Tensor] = {}
+
+    # -- fused path (fh_keyswitch_fused) -----------------------------------
+    def digit_table(self, level: int) -> torch.Tensor:
+        """[active digits, 16] int64 descriptors for fh_keyswitch_fused."""
+
+        hit = self._digit_tables.get(level)
+        if hit is not None:
+            return hit
+        words = []
+        for spec in self.layout.digit_specs(level):
+            t = self.rt.digit_tables[spec.prime_ids]
+            words.append(self._digit_words(t, spec.component_row_ids[0], spec.key_digit_index))
+        table = torch.tensor(words, dtype=torch.int64, device=self.rt.device)
+        self._digit_tables[level] = table
+        return table
+
+    def _digit_words(self, t, row_begin: int, key_digit: int,
+                     moddown_table: torch.Tensor | None = None) -> list[int]:
+        d = len(t.prime_ids)
+        multi = d > 1
+        # binary64 basis extension needs every digit below 2^45
+        f64_ok = int(all(self.config.moduli[i] < (1 << 45) for i in t.prime_ids))
+        return [
+            row_begin, d, key_digit,
+            t.normalizers.data_ptr() if multi else 0,
+            t.propagation.data_ptr() if multi else 0,
+            t.modulus_lo.data_ptr(), t.modulus_hi.data_ptr(), t.neg_inv_lo.data_ptr(),
+            t.neg_inv_hi.data_ptr(),
+            t.basis_extension.data_ptr() if multi else 0,
+            t.basis_extension.stride(0) if multi else 0,
+            t.basis_extension_std.data_ptr() if multi else 0,
+            f64_ok,
+            t.basis_extension_shoup.data_ptr() if multi else 0,
+            moddown_table.data_ptr() if moddown_table is not None else 0,
+            moddown_table.stride(0) if moddown_table is not None else 0,
+        ]
+
+    def p_digit_table(self) -> torch.Tensor:
+        """[16] descriptor of the P primes as one mixed-radix digit."""
+
+        hit = self.__dict__.get("_p_digit_table")
+        if hit is None:
+            words = self._digit_words(self.rt.p_digit, 0, 0, self.rt.moddown_binary64)
+            hit = torch.tensor(words, dtype=torch.int64, device=self.rt.device)
+            self._p_digit_table = hit
+        return hit
+
+    def _address_table(self, tensors: list[torch.Tensor], dev: torch.device) -> torch.Tensor:
+        """Device table of tensor addresses, cached: building it is a host-to-
+        device copy, which from pageable memory would stall the host."""
+
+        cache = self.__dict__.setdefault("_bases_cache", {})
+        tag = tuple(k.data_ptr() for k in tensors)
+        hit = cache.get(tag)
+        if hit is None or hit[1] != [id(k) for k in tensors]:
+            if len(cache) > 4096:
+                cache.clear()
+            table = torch.tensor(tag, dtype=torch.int64).pin_memory().to(dev, non_blocking=True)
+            # keep the tensors alive alongside their addresses
+            hit = (table, [id(k) for k in tensors], list(tensors))
+            cache[tag] = hit
+        return hit[0]
+
+    def key_switch_fused(self, component: torch.Tensor, key: KeySwitchKey | list[KeySwitchKey],
+                         level: int, base0: torch.Tensor | None = None,
+                         base1: torch.Tensor | None = None,
+                         galois_elements: list[int] | None = None) -> torch.Tensor:
+        """Whole key switch in six launches; returns [2, *batch, Q_level, N]
+        holding (base0 + corr0, base1 + corr1) canonically added, or the bare
+        corrections where a base is None.
+
+        With ``galois_elements`` (one per batch entry) the component and base0
+        are read through entry b's coefficient automorphism, i.e. this is the
+        key switch of a rotation applied to the unrotated ciphertext."""
+
+        from fhelium_amd import _native as nat
+
+        rt = self.rt
+        cfg = self.config
+        n = cfg.N
+        lq = cfg.num_q_primes - rt.row_start(level)
+        lqp = lq + cfg.num_p_primes
+        batch_shape = component.shape[:-2]
+        comp = component.reshape(-1, lq, n)
+        b = comp.size(0)
+        table = self.digit_table(level)
+        dnum = table.size(0)
+        dev = component.device
+        ws_digits = torch.empty((b, lq, n), dtype=torch.int64, device=dev)
+        ws_inter = torch.empty((b, dnum, lqp, n), dtype=torch.int64, device=dev)
+        ws_acc = torch.empty((2, b, lqp, n), dtype=torch.int64, device=dev)
+        out = torch.empty((2, b, lq, n), dtype=torch.int64, device=dev)
+        if isinstance(key, (list, tuple)):
+            if len(key) != b:
+                raise ValueError("one key per batch entry is required")
+            keys = [k.data for k in key]
+            ref = keys[0]
+            for k in keys:
+                if k.shape != ref.shape or k.stride() != ref.stride():
+                    raise ValueError("per-entry keys must share shape and strides")
+            bases = self._address_table(keys, dev)
+            key_ptr, bases_ptr = None, bases.data_ptr()
+        else:
+            ref = key.data
+            key_ptr, bases_ptr = ref.data_ptr(), None
+        if ref.dim() != 4 or ref.size(1) != 2 or ref.size(2) != cfg.total_num_primes or ref.size(3) != n:
+            raise ValueError("key must be [digit, 2, QP, N]")
+        gal_idx_ptr = gal_sgn_ptr = None
+        if galois_elements is not None:
+            if len(galois_elements) != b:
+                raise ValueError("one Galois element per batch entry is required")
+            tables = [rt.coefficient_galois(int(g)) for g in galois_elements]
+            gal_idx_ptr = self._address_table([t[0] for t in tables], dev).data_ptr()
+            gal_sgn_ptr = self._address_table([t[1] for t in tables], dev).data_ptr()
+        none = nat.fh_view3(None, 0, 0, 0)
+        view = lambda t: nat.v3(t.reshape(-1, lq, n)) if t is not None else none  # noqa: E731
+        nt = rt.ntt_tables
+        start = rt.row_start(level)
+        small = [q < SMALL_PRIME_LIMIT for q in cfg.moduli[start: start + lqp]]
+        small_rows = sum(small)
+        if any(small[small_rows:]):
+            raise ValueError("fused key switch needs the binary64 rows before the integer rows")
+        nat.call("fh_keyswitch_fused", nat.v3(comp), b, lq, lqp, start, cfg.logN, small_rows,
+                 table.data_ptr(), dnum, self.p_digit_table().data_ptr(), rt.p_inverse_q.data_ptr(),
+                 nat.t2(rt.rns_params), nt.rows.data_ptr(), nt.fwd.data_ptr(), nt.fwd_shoup.data_ptr(),
+                 nt.inv.data_ptr(), nt.inv_shoup.data_ptr(), key_ptr, bases_ptr,
+                 ref.stride(0), ref.stride(1), ref.stride(2), gal_idx_ptr, gal_sgn_ptr,
+                 ws_digits.data_ptr(), ws_inter.data_ptr(), ws_acc.data_ptr(), view(base0), view(base1),
+                 nat.v3(out[0]), nat.v3(out[1]), rt.ntt._mode, nat.stream_ptr(dev))
+        # Workspaces and the address tables may be released now: PyTorch's
+        # caching allocator only hands their blocks to later work on this
+        # same stream, which is ordered after these launches.
+        return out.reshape(2, *batch_shape, lq, n)
 
     def _mod_up(self, component: torch.Tensor, spec, level: int) -> torch.Tensor:
         """Digit rows -> mixed-radix digits -> [*, QP_level, N] Montgomery."""

@@ -128,6 +128,8 @@ class DigitTables:
     neg_inv_lo: torch.Tensor
     neg_inv_hi: torch.Tensor
     basis_extension: torch.Tensor      # [D-1, QP0]  M_r R^2 mod p_j over level-0 QP rows
+    basis_extension_std: torch.Tensor  # [D-1, QP0]  M_r R mod p_j (REDC of the above)
+    basis_extension_shoup: torch.Tensor  # [D-1, QP0] floor(std * 2^64 / p_j) as two's complement
 
 
 def _digit_tables(moduli: tuple[int, ...], prime_ids: tuple[int, ...], qp_ids: tuple[int, ...]) -> DigitTables:
@@ -145,6 +147,8 @@ def _digit_tables(moduli: tuple[int, ...], prime_ids: tuple[int, ...], qp_ids: t
     dest = [moduli[i] for i in qp_ids]
     r2 = [pow(R, 2, p) for p in dest]
     basis = [[(products[c] * r2[j]) % dest[j] for j in range(len(dest))] for c in range(d - 1)]
+    basis_std = [[(products[c] * R) % dest[j] for j in range(len(dest))] for c in range(d - 1)]
+    basis_shoup = [[_as_signed64((w << 64) // dest[j]) for j, w in enumerate(row)] for row in basis_std]
     ks = [(-pow(q, -1, R)) % R for q in src]
     as_t = lambda v, shape=None: torch.tensor(v, dtype=torch.int64).reshape(shape) if shape else torch.tensor(v, dtype=torch.int64)
     return DigitTables(
@@ -156,7 +160,42 @@ def _digit_tables(moduli: tuple[int, ...], prime_ids: tuple[int, ...], qp_ids: t
         neg_inv_lo=as_t([k & _HALF_MASK for k in ks]),
         neg_inv_hi=as_t([k >> HALF_BITS for k in ks]),
         basis_extension=as_t(basis, (d - 1, len(dest))) if d > 1 else torch.empty(0, len(dest), dtype=torch.int64),
+        basis_extension_std=(as_t(basis_std, (d - 1, len(dest))) if d > 1
+                             else torch.empty(0, len(dest), dtype=torch.int64)),
+        basis_extension_shoup=(as_t(basis_shoup, (d - 1, len(dest))) if d > 1
+                               else torch.empty(0, len(dest), dtype=torch.int64)),
     )
+
+
+def _as_signed64(v: int) -> int:
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
+# Binary64 ModDown split: a P digit c < 2^60 is taken as c_hi 2^30 + c_lo.
+MODDOWN_SPLIT_BITS = 30
+
+
+def moddown_binary64_table(moduli: tuple[int, ...], p_ids: tuple[int, ...],
+                           q_ids: tuple[int, ...]) -> torch.Tensor:
+    """[2P + 1, Q0] float64 rows for ModDown on Q rows below 2^46:
+    M_r mod q, 2^30 M_r mod q (M_r = prod_{t<r} p_t), then P^{-1} mod q.
+    Rows of larger primes hold zeros (they take the integer path)."""
+
+    p = [moduli[i] for i in p_ids]
+    prefix = [1]
+    for v in p[:-1]:
+        prefix.append(prefix[-1] * v)
+    p_product = math.prod(p)
+    out = torch.zeros((2 * len(p) + 1, len(q_ids)), dtype=torch.float64)
+    for j, qi in enumerate(q_ids):
+        q = moduli[qi]
+        if q >= (1 << 46):
+            continue
+        for r, m in enumerate(prefix):
+            out[r, j] = float(m % q)
+            out[len(p) + r, j] = float((m << MODDOWN_SPLIT_BITS) % q)
+        out[2 * len(p), j] = float(pow(p_product, -1, q))
+    return out
 
 
 class HostTables:
@@ -182,6 +221,15 @@ class HostTables:
         self.p_product_montgomery_q = torch.tensor(
             [(p_product * R) % moduli[i] for i in self.layout.prime_ids(0)], dtype=torch.int64
         )
+        # Fused ModDown tables: mixed-radix digits of the P residues (the P
+        # primes treated as one digit) extended onto every QP row, and
+        # P^{-1} mod q for the Q rows.  floor(X / P) = (X - (X mod P)) / P.
+        self.p_digit = _digit_tables(moduli, self.layout.p_prime_ids, qp0)
+        self.p_inverse_q = torch.tensor(
+            [pow(p_product, -1, moduli[i]) for i in self.layout.prime_ids(0)], dtype=torch.int64
+        )
+        self.moddown_binary64 = moddown_binary64_table(moduli, self.layout.p_prime_ids,
+                                                       self.layout.prime_ids(0))
 
     def _rescale_inverse(self, level: int) -> torch.Tensor:
         ids = self.layout.prime_ids(level, include_p=True)

@@ -30,6 +30,15 @@ class DeviceDigitTables:
     neg_inv_lo: torch.Tensor
     neg_inv_hi: torch.Tensor
     basis_extension: torch.Tensor  # [D-1, QP0]
+    basis_extension_std: torch.Tensor  # [D-1, QP0]
+    basis_extension_shoup: torch.Tensor  # [D-1, QP0]
+
+    @classmethod
+    def from_host(cls, key, t, dev) -> "DeviceDigitTables":
+        return cls(key, t.normalizers.to(dev), t.propagation.to(dev).contiguous(), t.modulus_lo.to(dev),
+                   t.modulus_hi.to(dev), t.neg_inv_lo.to(dev), t.neg_inv_hi.to(dev),
+                   t.basis_extension.to(dev).contiguous(), t.basis_extension_std.to(dev).contiguous(),
+                   t.basis_extension_shoup.to(dev).contiguous())
 
 
 class RnsRuntime:
@@ -51,15 +60,14 @@ class RnsRuntime:
                                          self.inverse_twiddles, config.logN)
         self.ntt = gntt.Gfx950NttBackend(self.ntt_tables, ntt_mode)
         self.digit_tables = {
-            key: DeviceDigitTables(
-                key, t.normalizers.to(dev), t.propagation.to(dev), t.modulus_lo.to(dev),
-                t.modulus_hi.to(dev), t.neg_inv_lo.to(dev), t.neg_inv_hi.to(dev),
-                t.basis_extension.to(dev))
-            for key, t in self.host.digit_tables.items()
+            key: DeviceDigitTables.from_host(key, t, dev) for key, t in self.host.digit_tables.items()
         }
         self.rescale_inverses = [v.to(dev) for v in self.host.rescale_inverses]
         self.moddown_inverses = [v.to(dev) for v in self.host.moddown_inverses]
         self.p_product_montgomery_q = self.host.p_product_montgomery_q.to(dev)
+        self.p_digit = DeviceDigitTables.from_host(self.layout.p_prime_ids, self.host.p_digit, dev)
+        self.p_inverse_q = self.host.p_inverse_q.to(dev)
+        self.moddown_binary64 = self.host.moddown_binary64.to(dev).contiguous()
         self._galois_cache: dict[int, tuple[torch.Tensor, torch.Tensor]] = {}
         self._ntt_galois_cache: dict[int, torch.Tensor] = {}
 

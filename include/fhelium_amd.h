@@ -150,6 +150,46 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
                                            int64_t key_stride_row, fh_table2 rns_params,
                                            int64_t key_row_start, int64_t batch, int64_t limbs,
                                            int64_t n, void* stream);
+/* A whole hybrid key switch of a batched component in six launches
+ * (decompose, ModUp fused into the forward NTT column pass, tile pass fused
+ * with the key MAC over every digit, inverse NTT, ModDown fused with the
+ * engine's final canonical addition).  Replaces the per-digit loop of
+ * fhelium/engine/hybrid_keyswitch.py:321-371 plus :530-568 and
+ * ckks_engine.py:1664-1667 / :3342, bit for bit; with Galois tables it also
+ * replaces the coefficient automorphism of a rotation (ckks_engine.py:3822-3897).
+ *   component        [batch, lq, N] canonical standard Q rows at `level`
+ *   small_rows       number of leading QP rows at `level` whose prime is
+ *                    below 2^46 (binary64 arithmetic; the rest are integer)
+ *   digit_table      [digit_count, 16] int64 digit descriptors (row_begin,
+ *                    rows, key_digit, addresses of the digit's mixed-radix
+ *                    and basis-extension tables, basis row stride, address
+ *                    of the standard-form basis table, binary64 flag)
+ *   p_digit_table    [16] int64 descriptor of the P primes taken as one
+ *                    digit (mixed-radix tables, extension onto QP0)
+ *   p_inverse_q      [Q0] P^{-1} mod q_j, standard form
+ *   rns_params / ntt_rows / twiddles: the full level-0 QP tables
+ *   key              shared key [digit, 2, QP0, N], or key_bases: [batch]
+ *                    device addresses of one such key per entry
+ *   galois_index_tables / galois_sign_tables: NULL, or [batch] device
+ *                    addresses of int32 source-index / int8 sign tables [N];
+ *                    component and base0 are then read through entry b's
+ *                    automorphism (a rotation's key switch)
+ *   workspace_*      digits [batch, lq, N], inter [batch, digit_count, lqp, N],
+ *                    acc [2, batch, lqp, N]
+ *   base0 / base1    optional canonical rows added to correction 0 / 1
+ *                    (data == NULL: none); out0 / out1 receive the results. */
+int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp, int64_t level,
+                       int64_t log_n, int64_t small_rows, const int64_t* digit_table,
+                       int64_t digit_count,
+                       const int64_t* p_digit_table, const int64_t* p_inverse_q,
+                       fh_table2 rns_params, const void* ntt_rows, const int64_t* forward_twiddles,
+                       const int64_t* forward_twiddles_shoup, const int64_t* inverse_twiddles,
+                       const int64_t* inverse_twiddles_shoup, const int64_t* key,
+                       const int64_t* key_bases, int64_t key_stride_digit, int64_t key_stride_comp,
+                       int64_t key_stride_row, const int64_t* galois_index_tables,
+                       const int64_t* galois_sign_tables, int64_t* workspace_digits,
+                       int64_t* workspace_inter, int64_t* workspace_acc, fh_view3 base0,
+                       fh_view3 base1, fh_view3 out0, fh_view3 out1, int32_t mode, void* stream);
 /* keyswitch_moddown_qp_to_q */
 int fh_ckks_moddown(fh_view3 out, fh_view3 q_residues, fh_view3 p_residues,
                     fh_table2 p_drop_inverses, fh_table2 rns_params, int64_t batch,

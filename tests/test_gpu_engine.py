@@ -173,3 +173,30 @@ def test_basic_flow_l7():
     rot = eng.rotate_by_step(prod, 1)
     r = eng.decrypt(rot).real
     np.testing.assert_allclose(r[1:33], (x * y), atol=2e-5)
+
+
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+@pytest.mark.parametrize("level", [0, 7, 33])
+def test_fused_keyswitch_matches_reference_sequence(mode, level):
+    """fh_keyswitch_fused == the reference's per-digit op sequence, bit for bit,
+    for a shared key and for per-entry keys, at logN=12 and logN=13."""
+    for log_n in (12, 13):
+        eng = CkksEngine(H.config(log_n), ntt_mode=mode, seed=21)
+        eng.generate_secret_key()
+        key = eng.generate_relinearization_key()
+        rng = np.random.default_rng(level)
+        comp = H.to_dev(H.random_residues(rng, 3, H.moduli_array(eng.config, level, eng.config.num_q_primes - level),
+                                          eng.config.N))
+        base = H.to_dev(H.random_residues(rng, 3, H.moduli_array(eng.config, level, eng.config.num_q_primes - level),
+                                          eng.config.N))
+        fused = eng.key_switcher.key_switch_fused(comp, key, level, base0=base)
+        c0, c1 = eng.key_switcher.apply_key_switch(comp, key, level)
+        from fhelium_amd import kernels as K
+        want0 = K.add_canonical(base, c0, eng.runtime.params(level))
+        np.testing.assert_array_equal(_np(fused[0]), _np(want0))
+        np.testing.assert_array_equal(_np(fused[1]), _np(c1))
+        keys = [eng.rotation_key(s) for s in (1, 2, 3)]
+        fused_multi = eng.key_switcher.key_switch_fused(comp, keys, level)
+        m0, m1 = eng.key_switcher.apply_key_switch(comp, keys, level)
+        np.testing.assert_array_equal(_np(fused_multi[0]), _np(m0))
+        np.testing.assert_array_equal(_np(fused_multi[1]), _np(m1))
