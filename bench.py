@@ -29,6 +29,10 @@ import torch
 
 METRIC = "rotations/s + HMult(relin+rescale)/s at logN=16 L=34, batched vs plain; p99"
 HBM_PEAK_GBS = 8000.0
+# Measured HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) per algorithmic
+# byte of the fused key switch on this workload; the rocprofv3 --pmc passes
+# and the arithmetic are recorded in profiles/keyswitch_fused_traffic.md.
+KS_TRAFFIC_BYTES_PER_ALG_BYTE = 4.103
 
 
 def parse_args():
@@ -42,6 +46,9 @@ def parse_args():
     p.add_argument("--skip-plain", action="store_true", help="time only the batched mode (profiling)")
     p.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    p.add_argument("--calibration-copy", action="store_true",
+                   help="run one 256 MiB device copy (torch.neg) before timing, as the byte "
+                        "reference for hardware-counter traffic figures")
     return p.parse_args()
 
 
@@ -109,10 +116,12 @@ def run_batched(eng, groups, lat_h, lat_r):
     return outs
 
 
-def timed(fn, steps: int, warmup: int):
+def timed(fn, steps: int, warmup: int, before_timing=None):
     for _ in range(warmup):
         fn(None, None)
     torch.cuda.synchronize()
+    if before_timing is not None:
+        before_timing()
     lat_h, lat_r = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -124,6 +133,25 @@ def timed(fn, steps: int, warmup: int):
 
 def percentile(values, q):
     return float(np.percentile(np.asarray(values), q)) if values else None
+
+
+def step_algorithmic_bytes(eng, groups) -> int:
+    """Compulsory HBM bytes of one step: per HMult at level l in a group of
+    B sharing the relinearization key, 4R in (two NTT-form inputs) + 2(R-1)
+    out + dnum 2 (39-l) / B key limbs; per rotation 2R in + 2R out + its own
+    key's dnum 2 (39-l) limbs (R = 35 - l Q rows, limb = N int64)."""
+
+    cfg = eng.config
+    limb = cfg.N * 8
+    total = 0
+    for lvl, g in groups.items():
+        b = len(g["steps"])
+        r = cfg.num_q_primes - lvl
+        qp = r + cfg.num_p_primes
+        dnum = len(eng.key_switcher.layout.digit_specs(lvl))
+        total += limb * (b * (4 * r + 2 * (r - 1)) + dnum * 2 * qp)
+        total += limb * b * (4 * r + dnum * 2 * qp)
+    return total
 
 
 def cpu_baseline(eng, seconds_budget: float = 30.0):
@@ -203,10 +231,24 @@ def main() -> int:
     else:
         ms_plain, plh, plr = timed(lambda h, r: run_plain(eng, groups, h, r), args.steps, args.warmup)
     log(f"plain   {ms_plain:.2f} ms/step")
-    eng.runtime.ntt.recorder = []
-    ms_batch, bh, br = timed(lambda h, r: run_batched(eng, groups, h, r), args.steps, args.warmup)
+    if args.calibration_copy:
+        src = torch.randint(0, 1 << 40, (1 << 25,), dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        log("calibration copy: 268435456 B read + 268435456 B written (torch.neg, int64 x 2^25)")
+        del src, dst
+    def start_recorders():
+        eng.runtime.ntt.recorder = []
+        eng.key_switcher.recorder = []
+
+    ms_batch, bh, br = timed(lambda h, r: run_batched(eng, groups, h, r), args.steps, args.warmup,
+                             start_recorders)
     rec = eng.runtime.ntt.recorder
+    ks_rec = eng.key_switcher.recorder
     eng.runtime.ntt.recorder = None
+    eng.key_switcher.recorder = None
     log(f"batched {ms_batch:.2f} ms/step")
 
     n_ops = 2 * args.count
@@ -217,7 +259,14 @@ def main() -> int:
     ntt_ms = sum(a.elapsed_time(b) for a, b, _ in rec)
     ntt_bytes = sum(nb for _, _, nb in rec)
     ntt_calls = len(rec) // args.steps if args.steps else 0
-    achieved = ntt_bytes / (ntt_ms * 1e-3) / 1e9 if ntt_ms > 0 else None
+    ntt_achieved = ntt_bytes / (ntt_ms * 1e-3) / 1e9 if ntt_ms > 0 else None
+    # timed region only: the recorders start after the warmup steps
+    ks_ms = sum(a.elapsed_time(b) for a, b, _ in ks_rec)
+    ks_bytes = sum(nb for _, _, nb in ks_rec)
+    ks_calls = len(ks_rec)
+    ks_achieved = ks_bytes / (ks_ms * 1e-3) / 1e9 if ks_ms > 0 else None
+    step_bytes = step_algorithmic_bytes(eng, groups)
+    step_achieved = step_bytes / (ms_batch * 1e-3) / 1e9
 
     result = {
         "metric": METRIC,
@@ -254,14 +303,28 @@ def main() -> int:
         "max_abs_error": max_err,
         "roofline": {
             "bound": "hbm",
-            "kernel": "NTT/INTT (fh_ntt_forward / fh_ntt_inverse, two-pass radix-16)",
-            "achieved": achieved,
+            "kernel": "fused hybrid key switch (fh_keyswitch_fused: k_ks_decompose, k_ks_modup_columns, "
+                      "k_ks_tiles_mac, k_inv_columns/k_inv_tiles, k_ks_moddown_add)",
+            "achieved": ks_achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "frac": (ks_achieved / HBM_PEAK_GBS) if ks_achieved else None,
+            "traffic": (KS_TRAFFIC_BYTES_PER_ALG_BYTE * ks_bytes / ks_calls
+                        if ks_calls and KS_TRAFFIC_BYTES_PER_ALG_BYTE else None),
+            "calls": ks_calls,
+            "avg_call_ms": ks_ms / ks_calls if ks_calls else None,
+            "algorithmic_bytes_per_call": (ks_bytes / ks_calls) if ks_calls else None,
+        },
+        "ntt_roofline": {
+            "bound": "hbm", "kernel": "fh_ntt_forward / fh_ntt_inverse (two-pass radix-16)",
+            "achieved": ntt_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (ntt_achieved / HBM_PEAK_GBS) if ntt_achieved else None,
             "calls_per_step": ntt_calls,
             "algorithmic_bytes_per_call": (ntt_bytes / len(rec)) if rec else None,
+        },
+        "step_roofline": {
+            "bound": "hbm", "algorithmic_bytes_per_step": step_bytes, "achieved": step_achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_achieved / HBM_PEAK_GBS,
         },
     }
     if args.cpu_baseline:

@@ -17,7 +17,7 @@ import threading
 import torch
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "_lib" / "libfhelium_amd.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class NativeExtensionError(RuntimeError):

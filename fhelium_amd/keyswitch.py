@@ -29,6 +29,9 @@ class HybridKeySwitcher:
         self.layout = runtime.layout
         self.config = runtime.config
         self._digit_tables: dict[int, torch.Tensor] = {}
+        # Optional list receiving (start_event, end_event, algorithmic_bytes)
+        # per fused key switch, recorded on the operand's stream.
+        self.recorder: list | None = None
 
     # -- fused path (fh_keyswitch_fused) -----------------------------------
     def digit_table(self, level: int) -> torch.Tensor:
@@ -146,6 +149,11 @@ class HybridKeySwitcher:
         none = nat.fh_view3(None, 0, 0, 0)
         view = lambda t: nat.v3(t.reshape(-1, lq, n)) if t is not None else none  # noqa: E731
         nt = rt.ntt_tables
+        rec = self.recorder
+        if rec is not None:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
         start = rt.row_start(level)
         small = [q < SMALL_PRIME_LIMIT for q in cfg.moduli[start: start + lqp]]
         small_rows = sum(small)
@@ -158,10 +166,24 @@ class HybridKeySwitcher:
                  ref.stride(0), ref.stride(1), ref.stride(2), gal_idx_ptr, gal_sgn_ptr,
                  ws_digits.data_ptr(), ws_inter.data_ptr(), ws_acc.data_ptr(), view(base0), view(base1),
                  nat.v3(out[0]), nat.v3(out[1]), rt.ntt._mode, nat.stream_ptr(dev))
+        if rec is not None:
+            ev1.record()
+            rec.append((ev0, ev1, self.algorithmic_bytes(b, lq, lqp, dnum, isinstance(key, (list, tuple)),
+                                                         (base0 is not None) + (base1 is not None))))
         # Workspaces and the address tables may be released now: PyTorch's
         # caching allocator only hands their blocks to later work on this
         # same stream, which is ordered after these launches.
         return out.reshape(2, *batch_shape, lq, n)
+
+    def algorithmic_bytes(self, batch: int, lq: int, lqp: int, dnum: int, per_entry_keys: bool,
+                          bases: int) -> int:
+        """Compulsory HBM bytes of one fused key switch: the component and
+        the bases read once, both outputs written once, and every key digit
+        (2 x QP rows) read once per distinct key."""
+
+        limb = self.config.N * 8
+        keys = batch if per_entry_keys else 1
+        return limb * (batch * lq * (1 + bases + 2) + keys * dnum * 2 * lqp)
 
     def _mod_up(self, component: torch.Tensor, spec, level: int) -> torch.Tensor:
         """Digit rows -> mixed-radix digits -> [*, QP_level, N] Montgomery."""

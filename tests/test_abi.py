@@ -1,0 +1,79 @@
+"""The C-ABI boundary on the CPU: the library loads without a GPU, exports
+every entry point include/fhelium_amd.h declares, and the package's ctypes
+table binds exactly those; the reference operator schemas are registered
+for the CUDA dispatch key only (no host fallback)."""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import re
+
+import pytest
+import torch
+
+from fhelium_amd import _native as nat
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "fhelium_amd.h"
+
+
+def _declared() -> set[str]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(fh_[a-z0-9_]+)\s*\(", text))
+
+
+@pytest.fixture(scope="module")
+def library():
+    if not nat.LIB_PATH.exists():
+        from fhelium_amd import build
+        build.build()
+    return nat.load()
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "fh_keyswitch_fused" in names and "fh_ntt_forward" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol(library):
+    lib = ctypes.CDLL(str(nat.LIB_PATH))
+    missing = [n for n in sorted(_declared()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_table_matches_header(library):
+    declared = _declared() - {"fh_last_error", "fh_abi_version", "fh_device_info"}
+    assert set(nat.exported_symbols()) == declared
+
+
+def test_abi_version(library):
+    assert library.fh_abi_version() == nat.ABI_VERSION
+
+
+def test_reference_schemas_registered():
+    from fhelium_amd import torch_ops
+    ops = torch_ops.registered_operators()
+    assert len(ops) == len(set(ops)) == 51
+    s = torch.ops.fhelium_ntt_ops.forward_ntt_montgomery_compact_keyswitch_accumulate_.default._schema
+    assert str(s) == (
+        "fhelium_ntt_ops::forward_ntt_montgomery_compact_keyswitch_accumulate_(Tensor(a!) "
+        "coefficient_digit_qp, Tensor forward_twiddles, Tensor rns_params, Tensor key_digit_qp, "
+        "Tensor(b!) accumulator0_qp, Tensor(c!) accumulator1_qp, int key_row_start) -> ()")
+    s = torch.ops.fhelium_ckks_ops.keyswitch_moddown_qp_to_q.default._schema
+    assert str(s) == (
+        "fhelium_ckks_ops::keyswitch_moddown_qp_to_q(Tensor q_residues, Tensor p_residues, "
+        "Tensor moddown_p_drop_inverses_montgomery, Tensor rns_params) -> Tensor")
+
+
+def test_host_tensors_have_no_fallback():
+    from fhelium_amd import torch_ops  # noqa: F401
+    x = torch.zeros((1, 2, 8), dtype=torch.int64)
+    params = torch.zeros((8, 2), dtype=torch.int64)
+    with pytest.raises(NotImplementedError):
+        torch.ops.fhelium_rns_ops.add_lazy(x, x, params)
+    with pytest.raises(NotImplementedError):
+        torch.ops.fhelium_ntt_ops.inverse_ntt_to_standard_compact_grouped_smem_(
+            x, torch.zeros((2, 8), dtype=torch.int64), params, 0)
