@@ -191,8 +191,9 @@ def cpu_baseline(eng, seconds_budget: float = 30.0):
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see the sharding note in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(the batch would shard across GPUs with no per-operation communication); "
+              "run with --gpus 1", file=sys.stderr)
         return 2
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm device", file=sys.stderr)
@@ -282,7 +283,7 @@ def main() -> int:
         "dtype": "int64",
         "data": "synthetic: seeded uniform[-1,1] slot vectors encrypted at scale 2^40, random keys",
         "config": {
-            "workload": "batched uneven (BASELINE configs[3]): {} ciphertext pairs, levels 0..32, "
+            "workload": "batched uneven CKKS evaluation: {} ciphertext pairs, levels 0..32, "
                         "HMult+relin+rescale then rotate by per-pair step 1..1024".format(args.count),
             "logN": 16, "levels": 34, "pairs": args.count, "ntt_mode": args.ntt_mode,
         },

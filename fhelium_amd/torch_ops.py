@@ -337,20 +337,37 @@ def _implementations() -> dict[str, dict[str, object]]:
     return {"fhelium_ntt_ops": ntt, "fhelium_rns_ops": rns, "fhelium_ckks_ops": ckks}
 
 
+def _existing_schema(ns: str, name: str) -> str | None:
+    try:
+        packet = getattr(getattr(torch.ops, ns), name)
+    except (AttributeError, RuntimeError):
+        return None
+    return str(packet.default._schema)
+
+
 def register() -> None:
-    """Define the schemas and register the CUDA-key implementations (once)."""
+    """Register the CUDA-key implementations (once).
+
+    Schemas that are not defined yet are defined here.  Where a library
+    already defined them (the reference's own extension with its CPU
+    kernels), the text must match and only the CUDA kernels are added."""
 
     with _lock:
         if _libraries:
             return
         impls = _implementations()
         for ns, schemas in SCHEMAS.items():
-            lib = torch.library.Library(ns, "FRAGMENT")
+            defs = torch.library.Library(ns, "FRAGMENT")
+            kernels = torch.library.Library(ns, "IMPL")
             for schema in schemas:
-                lib.define(schema)
                 name = schema.split("(")[0]
-                lib.impl(name, impls[ns][name], "CUDA")
-            _libraries.append(lib)
+                existing = _existing_schema(ns, name)
+                if existing is None:
+                    defs.define(schema)
+                elif existing != f"{ns}::{schema}":
+                    raise RuntimeError(f"{ns}::{name} is already defined with a different schema: {existing}")
+                kernels.impl(name, impls[ns][name], "CUDA")
+            _libraries.extend((defs, kernels))
 
 
 def registered_operators() -> list[str]:

@@ -142,3 +142,45 @@ def test_host_tables_match_closed_form_fixtures():
     assert t.normalizers.tolist() == mr["normalizers"]
     assert t.propagation.tolist() == mr["propagation"]
     assert t.basis_extension.tolist() == mr["basis_extension"]
+
+
+def test_remaining_elementwise_closed_forms():
+    """Row scalars, add_lazy_with_twice_modulus, centered lift and the three
+    compressed plaintext layouts against big-integer closed forms."""
+    rng = np.random.default_rng(43)
+    case = GOLDEN["ntt"][0]
+    q = A(case["moduli"])
+    params = A(case["params"])
+    n = 64
+    qq = q.reshape(1, -1, 1)
+    R = 1 << 62
+    rinv = np.array([pow(R, -1, int(m)) for m in q], dtype=object).reshape(1, -1, 1)
+    a = np.stack([rng.integers(0, 2 * int(m), n) for m in q])[None]
+    ac = a % qq
+    s = np.array([rng.integers(0, int(m)) for m in q], dtype=np.int64)
+    for canonical in (False, True):
+        got = O.row_scalars(a, s, params, canonical)
+        want = (a.astype(object) * s.reshape(1, -1, 1) * rinv % qq).astype(np.int64)
+        np.testing.assert_array_equal(got, want)
+    b = np.stack([rng.integers(0, 2 * int(m), n) for m in q])[None]
+    got = O.add_lazy_twice(a, b, 2 * q)
+    assert (got >= 0).all() and (got < 2 * qq).all()
+    np.testing.assert_array_equal(got % qq, (a + b) % qq)
+    small = rng.integers(-(1 << 20), 1 << 20, size=(2, n))
+    got = O.lift(small, 2 * q)
+    assert (got >= 0).all() and (got < 2 * qq).all()
+    np.testing.assert_array_equal(got % qq, small[:, None, :] % qq)
+    support = 16
+    repeat = n // support
+    pt = np.stack([rng.integers(0, int(m), support) for m in q])[None]
+    implicit = np.array([rng.integers(0, int(m)) for m in q], dtype=np.int64)
+    idx = np.arange(n)
+    expanded = {
+        "cyclic": pt[:, :, idx % support],
+        "contiguous": pt[:, :, idx // repeat],
+        "strided": np.where(idx % repeat == 0, pt[:, :, idx // repeat], implicit.reshape(1, -1, 1)),
+    }
+    for layout, p in expanded.items():
+        got = O.add_plaintext(layout, ac, pt, params, implicit if layout == "strided" else None)
+        want = ((ac.astype(object) + p * rinv) % qq).astype(np.int64)
+        np.testing.assert_array_equal(got, want, err_msg=layout)
