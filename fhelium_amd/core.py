@@ -94,6 +94,27 @@ class Ciphertext:
             raise CkksStateError(f"expected basis {modulus_basis}, got {self.modulus_basis}")
 
 
+class ScaleMismatchError(ValueError):
+    """Operands whose binary64 scales are not exactly equal (errors.py)."""
+
+
+@dataclass
+class Plaintext:
+    """Operation-ready RNS plaintext ``[*batch, limb, N]`` in Montgomery form,
+    coefficient or NTT domain (fhelium/core/plaintext.py)."""
+
+    data: torch.Tensor
+    level: int
+    scale: float
+    polynomial_domain: str = "coefficient"
+    residue_representation: str = "montgomery"
+    modulus_basis: str = "Q"
+
+    @property
+    def batch_shape(self) -> tuple[int, ...]:
+        return tuple(self.data.shape[:-2])
+
+
 @dataclass
 class SecretKey:
     """s on every QP row, NTT domain, Montgomery form: [QP, N]."""

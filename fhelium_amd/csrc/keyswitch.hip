@@ -3,8 +3,9 @@
 // Same mathematics as the reference's streaming key switch
 // (fhelium/engine/hybrid_keyswitch.py:321-371, 530-568 with the native ops
 // csrc/ops/rns/cpu/rns_arithmetic_cpu.cpp:482-700 and
-// csrc/ops/ckks/cpu/ckks_cpu.cpp:474-708), reorganised into six launches
-// instead of ~100:
+// csrc/ops/ckks/cpu/ckks_cpu.cpp:474-708), reorganised into eight launches
+// instead of ~100 (ModUp and the tile pass launch once for the binary64 rows
+// and once for the integer rows):
 //
 //   1. k_ks_decompose     Garner digits of every active hybrid digit (one
 //                         launch for all digits); optionally reads its input
@@ -15,9 +16,11 @@
 //   3. k_ks_tiles_mac     forward NTT tile pass of every digit fused with the
 //                         key multiply-accumulate; both accumulators stay in
 //                         registers across all digits and are written once.
-//                         Blocks are remapped so one XCD walks the batch
-//                         entries of one (row, tile) consecutively and re-reads
-//                         the key tile from its L2.
+//                         8 coefficients per thread, the tile's twiddles
+//                         staged in LDS, each digit's key loads issued with
+//                         its tile loads.  Blocks are remapped so one XCD
+//                         walks the batch entries of one (row, tile)
+//                         consecutively and re-reads the key tile from its L2.
 //   4. inverse NTT of both accumulators (ntt.hip, working-format input).
 //   5. k_ks_moddown_add   floor(x / P) mod q_i computed as
 //                         (x - (x mod P)) * P^{-1} with x mod P from the
@@ -48,7 +51,7 @@ using namespace ::fh::ntt;
 
 constexpr int kMaxDigitRows = 8;
 constexpr int kMaxP = 8;
-constexpr int kModDownRowsPerThread = 4;
+constexpr int kModDownRowsPerThread = 8;
 
 // One active hybrid digit (RnsDigitSpec plus device table addresses),
 // packed by the host as 16 int64 words.  Also used for the P primes.
@@ -94,6 +97,18 @@ struct KsArgs {
   const int64_t* gal_idx;   // per-entry int32 gather table addresses [B] or null
   const int64_t* gal_sign;  // per-entry int8 sign table addresses [B]
 };
+
+// Block i runs on XCD i % 8; give each XCD one contiguous range of work
+// items (batch fastest) so consecutive items on an XCD share key tiles in
+// its L2.  The map is a bijection of [0, total).
+__device__ __forceinline__ int64_t xcd_remap(int64_t i, int64_t total) {
+  constexpr int64_t kXcds = 8;
+  const int64_t x = i % kXcds;
+  const int64_t s = i / kXcds;
+  const int64_t q = total / kXcds;
+  const int64_t rem = total % kXcds;
+  return x * q + (x < rem ? x : rem) + s;
+}
 
 // Table addresses read from device memory are generic pointers to the
 // compiler, which would then emit FLAT loads; those also count against the
@@ -161,10 +176,13 @@ __device__ __forceinline__ void garner(const KsDigit& d, int64_t* dg) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_ks_decompose(KsArgs a) {
   const int64_t n = int64_t{1} << a.log_n;
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const KsDigit& d = a.dig[blockIdx.y];
-  const int64_t b = blockIdx.z;
+  // One XCD walks all coefficient chunks of an (entry, digit): a Galois
+  // gather then touches each source row in one L2 only.
+  const int64_t chunks = n / kBlock;
+  const int64_t w = xcd_remap(blockIdx.x, chunks * a.dnum * a.batch);
+  const int64_t i = (w % chunks) * kBlock + threadIdx.x;
+  const KsDigit& d = a.dig[(w / chunks) % a.dnum];
+  const int64_t b = w / (chunks * a.dnum);
   const int rows = static_cast<int>(d.rows);
   int64_t dg[kMaxDigitRows];
 #pragma unroll
@@ -335,6 +353,26 @@ __device__ __forceinline__ void mac_f64(double& acc, double y, int64_t key, doub
   acc += f64_mulmod_signed(static_cast<double>(key), y, q, qinv);
 }
 
+
+// Tile pass with 8 elements per thread (32 threads per 256-element tile,
+// 8 tiles per block): rounds of 3, 3 and 2 stages exchanged through LDS.
+// Each tile's 255 twiddles (they do not depend on the digit) and the key
+// digit indices are staged in LDS once per block, so the digit loop issues
+// only the tile and key loads -- together, at the top of the iteration --
+// and the per-thread state is small enough for 4 waves per SIMD.
+constexpr int kMacThreads = 32;                     // threads per tile
+constexpr int kMacTiles = kBlock / kMacThreads;     // tiles per block
+constexpr int kMacExchange = kMacTiles * kTilePitch;
+constexpr int kMacMaxDigits = 16;
+
+// Element of the tile held in slot k by thread t in each round.
+__device__ __forceinline__ int mac_e1(int t, int k) { return t + 32 * k; }
+__device__ __forceinline__ int mac_e2(int t, int k) { return ((t >> 2) << 5) | (k << 2) | (t & 3); }
+__device__ __forceinline__ int mac_e3(int t, int k) { return 8 * t + k; }
+// Slot of the twiddle of the pair whose lower element is e at tile stage st:
+// stage st uses 2^st twiddles, stored after those of the earlier stages.
+__device__ __forceinline__ int mac_tw_slot(int st, int e) { return (1 << st) - 1 + (e >> (8 - st)); }
+
 template <class P>
 __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g, int64_t j,
                                           int64_t b, int64_t* lds) {
@@ -343,119 +381,146 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
   const int64_t n = int64_t{1} << a.log_n;
   const int64_t grow = a.level + j;
   const NttRow& row = a.rows[grow];
-  const int64_t* tw_row = a.fwd_w + grow * n;
-  const int64_t* twp_row = a.fwd_wp + grow * n;
-  const int m = threadIdx.x / 16;
-  const int t = threadIdx.x % 16;
-  const int r = static_cast<int>(g) * kTilesPerBlock + m;
-  int64_t* sh = lds + m * kTilePitch;
+  const int m = threadIdx.x / kMacThreads;
+  const int t = threadIdx.x % kMacThreads;
+  const int r = static_cast<int>(g) * kMacTiles + m;
   const int s0 = a.log_n - kTileLog;
+  int64_t* sh = lds + m * kTilePitch;
+  int64_t* tww = lds + kMacExchange + m * kTile;
+  int64_t* twp = tww + kMacTiles * kTile;  // integer rows only
+  int64_t* key_digit = lds + kMacExchange + kMacTiles * kTile * (kF64 ? 1 : 2);
+  {
+    const int64_t* w_row = a.fwd_w + grow * n;
+    const int64_t* wp_row = a.fwd_wp + grow * n;
+    for (int idx = t; idx < kTile - 1; idx += kMacThreads) {
+      const int st = 31 - __builtin_clz(idx + 1);
+      const int64_t gi = (int64_t{1} << (s0 + st)) + (static_cast<int64_t>(r) << st) + (idx + 1 - (1 << st));
+      tww[idx] = w_row[gi];
+      if constexpr (!kF64) twp[idx] = wp_row[gi];
+    }
+    if (threadIdx.x < a.dnum) key_digit[threadIdx.x] = a.dig[threadIdx.x].key_digit;
+  }
+  __syncthreads();
+  auto tw_at = [&](int st, int e) {
+    const int slot = mac_tw_slot(st, e);
+    return p.tw_make(tww[slot], kF64 ? 0 : twp[slot]);
+  };
   Global<int64_t>* key = global(a.key_bases ? reinterpret_cast<const int64_t*>(a.key_bases[b]) : a.key);
   const int64_t tile_off = static_cast<int64_t>(r) * kTile;
   const Mont mont = load_mont(a.params, grow);
-  double facc0[16], facc1[16];
-  uint64_t uacc0[16], uacc1[16];
+  double facc0[8], facc1[8];
+  uint64_t uacc0[8], uacc1[8];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
+  for (int k = 0; k < 8; ++k) {
     facc0[k] = facc1[k] = 0.0;
     uacc0[k] = uacc1[k] = 0;
   }
-  for (int64_t di = 0; di < a.dnum; ++di) {
-    const KsDigit& d = a.dig[di];
-    const int64_t* tile = a.inter + ((b * a.dnum + di) * a.lqp + j) * n + tile_off;
-    Global<int64_t>* k0 = key + d.key_digit * a.key_sd + grow * a.key_sr + tile_off;
+  const int64_t* tile = a.inter + ((b * a.dnum) * a.lqp + j) * n + tile_off;
+  const int64_t digit_stride = a.lqp * n;
+#pragma nounroll
+  for (int64_t di = 0; di < a.dnum; ++di, tile += digit_stride) {
+    Global<int64_t>* k0 = key + key_digit[di] * a.key_sd + grow * a.key_sr + tile_off;
     Global<int64_t>* k1 = k0 + a.key_sc;
-    // The twiddles do not depend on the digit; re-read them (L1/L2 hits)
-    // every iteration instead of keeping ~30 of them live in registers
-    // across the loop, which would halve the occupancy.
-    const int64_t* tw_opaque = tw_row;
-    const int64_t* twp_opaque = twp_row;
-    asm volatile("" : "+v"(tw_opaque), "+v"(twp_opaque));
-    const int64_t* tw = (const int64_t*)global(tw_opaque);
-    const int64_t* twp = (const int64_t*)global(twp_opaque);
-    V v[16];
+    V v[8];
+    int64_t w0[8], w1[8];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = p.unmid(tile[t + 16 * k]);
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(tile[mac_e1(t, k)]);
+    // Binary64 rows issue the key loads with the tile loads; the integer
+    // rows (whose products need many more registers) load them at the MAC.
+    if constexpr (kF64) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int half = 8 >> s;
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k & half) continue;
-        p.fwd(v[k], v[k + half], p.tw(tw, twp, (1 << (s0 + s)) + (r << s) + (k >> (4 - s))));
+      for (int k = 0; k < 8; ++k) {
+        w0[k] = k0[mac_e3(t, k)];
+        w1[k] = k1[mac_e3(t, k)];
       }
     }
-    // Each thread writes exactly the words it read last time round (the
-    // final transposed read below), so no barrier is needed before this.
+    // Round 1: tile stages 0..2 pair element bits 7..5 = bits 2..0 of k.
 #pragma unroll
-    for (int k = 0; k < 16; ++k) sh[tile_pad(t + 16 * k)] = p.mid(v[k]);
+    for (int st = 0; st < 3; ++st) {
+      const int half = 4 >> st;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half], tw_at(st, mac_e1(t, k)));
+      }
+    }
+    // The previous digit's last reads of this buffer must be done.
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = p.unmid(sh[17 * t + k]);
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
+    __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int half = 8 >> s;
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
+    // Round 2: tile stages 3..5 pair element bits 4..2 = bits 2..0 of k.
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
+    for (int st = 3; st < 6; ++st) {
+      const int half = 1 << (5 - st);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
         if (k & half) continue;
-        p.fwd(v[k], v[k + half],
-              p.tw(tw, twp, (1 << (s0 + 4 + s)) + (r << (4 + s)) + (t << s) + (k >> (4 - s))));
+        p.fwd(v[k], v[k + half], tw_at(st, mac_e2(t, k)));
+      }
+    }
+    // Each thread rewrites the words it just read: no barrier needed.
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
+    // Round 3: tile stages 6..7 pair element bits 1..0 = bits 1..0 of k.
+#pragma unroll
+    for (int st = 6; st < 8; ++st) {
+      const int half = 1 << (7 - st);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half], tw_at(st, mac_e3(t, k)));
       }
     }
 #pragma unroll
-    for (int k = 0; k < 16; ++k) sh[17 * t + k] = p.mid(v[k]);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const V y = p.unmid(sh[tile_pad(t + 16 * k)]);
-      const int64_t w0 = k0[t + 16 * k];
-      const int64_t w1 = k1[t + 16 * k];
+    for (int k = 0; k < 8; ++k) {
       if constexpr (kF64) {
-        mac_f64(facc0[k], y, w0, row.qd, row.qinv);
-        mac_f64(facc1[k], y, w1, row.qd, row.qinv);
+        mac_f64(facc0[k], v[k], w0[k], row.qd, row.qinv);
+        mac_f64(facc1[k], v[k], w1[k], row.qd, row.qinv);
       } else {
-        const int64_t yi = static_cast<int64_t>(y);
+        const int64_t yi = static_cast<int64_t>(v[k]);
+        w0[k] = k0[mac_e3(t, k)];
+        w1[k] = k1[mac_e3(t, k)];
         uacc0[k] = static_cast<uint64_t>(
-            add_lazy(static_cast<int64_t>(uacc0[k]), mont_mul(yi, w0, mont), mont.twice_q));
+            add_lazy(static_cast<int64_t>(uacc0[k]), mont_mul(yi, w0[k], mont), mont.twice_q));
         uacc1[k] = static_cast<uint64_t>(
-            add_lazy(static_cast<int64_t>(uacc1[k]), mont_mul(yi, w1, mont), mont.twice_q));
+            add_lazy(static_cast<int64_t>(uacc1[k]), mont_mul(yi, w1[k], mont), mont.twice_q));
       }
     }
-    // The next digit's first LDS write targets this thread's own words
-    // (tile_pad(t + 16k)), which only this thread reads above.
   }
   int64_t* out0 = a.acc + ((0 * a.batch + b) * a.lqp + j) * n + tile_off;
   int64_t* out1 = a.acc + ((1 * a.batch + b) * a.lqp + j) * n + tile_off;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
+  for (int k = 0; k < 8; ++k) {
     if constexpr (kF64) {
-      out0[t + 16 * k] = __double_as_longlong(facc0[k]);
-      out1[t + 16 * k] = __double_as_longlong(facc1[k]);
+      out0[mac_e3(t, k)] = __double_as_longlong(facc0[k]);
+      out1[mac_e3(t, k)] = __double_as_longlong(facc1[k]);
     } else {
-      out0[t + 16 * k] = static_cast<int64_t>(uacc0[k]);
-      out1[t + 16 * k] = static_cast<int64_t>(uacc1[k]);
+      out0[mac_e3(t, k)] = static_cast<int64_t>(uacc0[k]);
+      out1[mac_e3(t, k)] = static_cast<int64_t>(uacc1[k]);
     }
   }
 }
 
-// Block i runs on XCD i % 8; give each XCD one contiguous range of work
-// items (batch fastest) so consecutive items on an XCD share key tiles in
-// its L2.  The map is a bijection of [0, total).
-__device__ __forceinline__ int64_t xcd_remap(int64_t i, int64_t total) {
-  constexpr int64_t kXcds = 8;
-  const int64_t x = i % kXcds;
-  const int64_t s = i / kXcds;
-  const int64_t q = total / kXcds;
-  const int64_t rem = total % kXcds;
-  return x * q + (x < rem ? x : rem) + s;
+template <bool kSmall>
+constexpr int mac_lds_words() {
+  return kMacExchange + kMacTiles * kTile * (kSmall ? 1 : 2) + kMacMaxDigits;
 }
+constexpr int kMacTilesPerBlock = kMacTiles;
 
 // One launch per arithmetic path (binary64 rows, then the integer rows) so
 // each kernel is register-allocated for its own path only.
 template <int MODE, bool kSmall>
-__global__ __launch_bounds__(kBlock) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
+// Register budgets: 4 waves per SIMD for the binary64 rows (a few spilled
+// dwords cost less than the lost occupancy), 2+ for the integer rows.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
                                                          int64_t j_count) {
-  __shared__ int64_t lds[kTilesPerBlock * kTilePitch];
+  __shared__ int64_t lds[mac_lds_words<kSmall>()];
   const int64_t total = groups * j_count * a.batch;
   const int64_t w = xcd_remap(blockIdx.x, total);
   const int64_t b = w % a.batch;
@@ -494,70 +559,106 @@ struct ModDownArgs {
 };
 
 __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
+  constexpr int kRows = kModDownRowsPerThread;
   const int64_t n = int64_t{1} << a.log_n;
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int64_t c = blockIdx.z / a.batch;
-  const int64_t b = blockIdx.z % a.batch;
+  // XCD-contiguous work items: all coefficient chunks of one (accumulator,
+  // entry, row group) run on one XCD (see k_ks_decompose).
+  const int64_t chunks = n / kBlock;
+  const int64_t groups = (a.lq + kRows - 1) / kRows;
+  const int64_t w = xcd_remap(blockIdx.x, chunks * groups * 2 * a.batch);
+  const int64_t i = (w % chunks) * kBlock + threadIdx.x;
+  const int64_t group = (w / chunks) % groups;
+  const int64_t cb = w / (chunks * groups);
+  const int64_t c = cb / a.batch;
+  const int64_t b = cb % a.batch;
   const int p_count = static_cast<int>(a.lqp - a.lq);
   const int64_t* x = a.acc + (c * a.batch + b) * a.lqp * n + i;
-  // Mixed-radix digits of X mod P (X_P = sum_r c_r prod_{t<r} p_t).
+  const View3& base = c == 0 ? a.base0 : a.base1;
+  const bool has_base = (c == 0 ? a.has_base0 : a.has_base1) != 0;
+  const View3& out = c == 0 ? a.out0 : a.out1;
+  const int64_t row_begin = group * kRows;
+  // Galois gather of the base (c0 of a rotation): one source index and sign
+  // per coefficient, shared by every row.
+  int64_t src_i = i;
+  bool negate = false;
+  if (c == 0 && a.gal_idx != nullptr) {
+    src_i = global(reinterpret_cast<const int32_t*>(a.gal_idx[b]))[i];
+    negate = global(reinterpret_cast<const int8_t*>(a.gal_sign[b]))[i] == -1;
+  }
+  // Issue every load of the thread first: P rows, Q rows, base rows.
   int64_t dg[kMaxDigitRows];
 #pragma unroll
   for (int r = 0; r < kMaxDigitRows; ++r) dg[r] = r < p_count ? x[(a.lq + r) * n] : 0;
+  int64_t xv[kRows], bv[kRows];
+#pragma unroll
+  for (int rr = 0; rr < kRows; ++rr) {
+    const int64_t row = row_begin + rr;
+    xv[rr] = row < a.lq ? x[row * n] : 0;
+    bv[rr] = (has_base && row < a.lq) ? base.at(b, row, src_i) : 0;
+  }
+  // Mixed-radix digits of X mod P (X_P = sum_r c_r prod_{t<r} p_t), each
+  // split as c_hi 2^30 + c_lo (both exact in binary64).
   garner(a.pdig, dg);
-  // Each digit split as c_hi 2^30 + c_lo, both exact in binary64.
   double dlo[kMaxP], dhi[kMaxP];
 #pragma unroll
   for (int r = 0; r < kMaxP; ++r) {
     dlo[r] = static_cast<double>(static_cast<uint32_t>(dg[r] & ((int64_t{1} << 30) - 1)));
     dhi[r] = static_cast<double>(static_cast<uint32_t>(dg[r] >> 30));
   }
-  const View3& base = c == 0 ? a.base0 : a.base1;
-  const int has_base = c == 0 ? a.has_base0 : a.has_base1;
-  const int64_t* gal_idx = c == 0 ? a.gal_idx : nullptr;
-  const View3& out = c == 0 ? a.out0 : a.out1;
   const int64_t q0 = a.pdig.moddown_stride;
-  const int64_t row_begin = static_cast<int64_t>(blockIdx.y) * kModDownRowsPerThread;
-  for (int rr = 0; rr < kModDownRowsPerThread; ++rr) {
-    const int64_t row = row_begin + rr;
-    if (row >= a.lq) break;
-    const int64_t grow = a.level + row;
-    const NttRow& nr = a.rows[grow];
-    int64_t value;
-    if (nr.small) {
-      // (x - X_P) P^{-1} mod q with X_P mod q = sum_r c_lo M_r + c_hi 2^30 M_r,
-      // all in signed binary64 (|sum| < 4.2q), canonicalised once.
-      const double q = nr.qd, qinv = nr.qinv;
-      Global<double>* tab = global(a.pdig.moddown_f64) + grow;
-      double xp = 0.0;
-      for (int r = 0; r < p_count; ++r) {
-        xp += f64_mulmod_signed(tab[r * q0], dlo[r], q, qinv);
-        xp += f64_mulmod_signed(tab[(p_count + r) * q0], dhi[r], q, qinv);
-      }
-      const double diff = u64_to_f64(static_cast<uint64_t>(x[row * n])) - xp;
-      double res = f64_mulmod_signed(tab[2 * p_count * q0], diff, q, qinv);
-      res = res < 0.0 ? res + q : res;
-      value = static_cast<int64_t>(f64_to_u64(res));
-    } else {
-      const Mont m = load_mont(a.params, grow);
-      // (X_P R) mod q by the reference basis-extension formula, then
-      // ((x R) - (X_P R)) * P^{-1} with REDC = (x - X_P) P^{-1} mod q.
-      int64_t xp = mont_mul(dg[0], m.r2, m);
-      for (int r = 1; r < p_count; ++r)
-        xp = add_lazy(xp, mont_mul(dg[r], global(a.pdig.basis)[(r - 1) * a.pdig.basis_stride + grow], m),
-                      m.twice_q);
-      const int64_t xr = mont_mul(x[row * n], m.r2, m);
-      value = mont_mul(sub_lazy(xr, xp, m.twice_q), a.p_inverse[grow], m);
-    }
+  // Final canonical addition of the (possibly gathered) base, as
+  // apply_coefficient_galois_automorphism + add_canonical compute it.
+  auto finish = [&](int rr, int64_t row, int64_t value, int64_t twice_q) {
     if (has_base) {
-      const int64_t twice_q = static_cast<int64_t>(nr.twice_q);
-      const int64_t bv = read_maybe_galois(base, gal_idx, a.gal_sign, b, row, i, twice_q);
-      int64_t sum = bv + value;
+      int64_t bval = bv[rr];
+      if (a.gal_idx != nullptr && c == 0) {
+        if (negate) bval = -bval;
+        bval = canonicalize(shift_positive(bval, twice_q), twice_q);
+      }
+      int64_t sum = bval + value;
       if (sum >= twice_q) sum -= twice_q;
       value = canonicalize(sum, twice_q);
     }
     out.at(b, row, i) = value;
+  };
+#pragma unroll
+  for (int rr = 0; rr < kRows; ++rr) {
+    const int64_t row = row_begin + rr;
+    if (row >= a.lq) break;
+    const int64_t grow = a.level + row;
+    const NttRow& nr = a.rows[grow];
+    if (!nr.small) continue;
+    // (x - X_P) P^{-1} mod q with X_P mod q = sum_r c_lo M_r + c_hi 2^30 M_r,
+    // all in signed binary64 (|sum| < 4.2q), canonicalised once.
+    const double q = nr.qd, qinv = nr.qinv;
+    Global<double>* tab = global(a.pdig.moddown_f64) + grow;
+    double xp = 0.0;
+#pragma unroll
+    for (int r = 0; r < kMaxP; ++r) {
+      if (r >= p_count) break;
+      xp += f64_mulmod_signed(tab[r * q0], dlo[r], q, qinv);
+      xp += f64_mulmod_signed(tab[(p_count + r) * q0], dhi[r], q, qinv);
+    }
+    const double diff = u64_to_f64(static_cast<uint64_t>(xv[rr])) - xp;
+    double res = f64_mulmod_signed(tab[2 * p_count * q0], diff, q, qinv);
+    res = res < 0.0 ? res + q : res;
+    finish(rr, row, static_cast<int64_t>(f64_to_u64(res)), static_cast<int64_t>(nr.twice_q));
+  }
+  // Rows of primes above 2^46 (the base prime): the integer formula.
+  for (int rr = 0; rr < kRows; ++rr) {
+    const int64_t row = row_begin + rr;
+    if (row >= a.lq) break;
+    const int64_t grow = a.level + row;
+    if (a.rows[grow].small) continue;
+    const Mont m = load_mont(a.params, grow);
+    // (X_P R) mod q by the reference basis-extension formula, then
+    // ((x R) - (X_P R)) * P^{-1} with REDC = (x - X_P) P^{-1} mod q.
+    int64_t xp = mont_mul(dg[0], m.r2, m);
+    for (int r = 1; r < p_count; ++r)
+      xp = add_lazy(xp, mont_mul(dg[r], global(a.pdig.basis)[(r - 1) * a.pdig.basis_stride + grow], m),
+                    m.twice_q);
+    const int64_t xr = mont_mul(x[row * n], m.r2, m);
+    finish(rr, row, mont_mul(sub_lazy(xr, xp, m.twice_q), a.p_inverse[grow], m), m.twice_q);
   }
 }
 
@@ -570,7 +671,8 @@ using namespace fh;
 FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp,
                               int64_t level, int64_t log_n, int64_t small_rows,
                               const int64_t* digit_table,
-                              int64_t digit_count, const int64_t* p_digit_table,
+                              int64_t digit_count,
+                              const int64_t* p_digit_table,
                               const int64_t* p_inverse_q, fh_table2 rns_params, const void* ntt_rows,
                               const int64_t* forward_twiddles, const int64_t* forward_twiddles_shoup,
                               const int64_t* inverse_twiddles, const int64_t* inverse_twiddles_shoup,
@@ -584,7 +686,7 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   FH_CHECK_ARG(batch >= 1 && batch <= 32767, "fused key switch batch %lld out of range",
                (long long)batch);
   FH_CHECK_ARG(lq >= 1 && lqp > lq && lqp - lq <= kMaxP, "bad Q/QP row counts");
-  FH_CHECK_ARG(digit_count >= 1 && digit_count * batch <= 65535, "bad digit count");
+  FH_CHECK_ARG(digit_count >= 1 && digit_count <= 16 && digit_count * batch <= 65535, "bad digit count");
   FH_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (exact) or 1 (fast)");
   FH_CHECK_ARG(component.data && digit_table && p_digit_table && p_inverse_q && rns_params.data &&
                    ntt_rows && forward_twiddles && forward_twiddles_shoup && inverse_twiddles &&
@@ -621,16 +723,14 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   a.gal_idx = galois_index_tables;
   a.gal_sign = galois_sign_tables;
 
-  hipLaunchKernelGGL(k_ks_decompose,
-                     dim3(static_cast<uint32_t>(ceil_div(n, kBlock)), static_cast<uint32_t>(digit_count),
-                          static_cast<uint32_t>(batch)),
-                     dim3(kBlock), 0, s, a);
-  FH_CHECK_LAUNCH("key switch decompose");
-
-  // Rows below 2^46 take the binary64 path (the same rule as the NTT row
-  // records); they precede the integer rows in the QP chain.
   FH_CHECK_ARG(small_rows >= 0 && small_rows <= lqp, "bad binary64 row count");
   const int s1 = static_cast<int>(log_n) - kTileLog;
+  // n >= 4096 is a multiple of the block size; grids are flat and remapped.
+  const int64_t dec_blocks = (n / kBlock) * digit_count * batch;
+  FH_CHECK_ARG(dec_blocks <= 0x7fffffff, "key switch grid too large");
+  hipLaunchKernelGGL(k_ks_decompose, dim3(static_cast<uint32_t>(dec_blocks)), dim3(kBlock), 0, s, a);
+  FH_CHECK_LAUNCH("key switch decompose");
+
   const int T = (1 << s1) / 16;
   const int C = kBlock / T;
   for (int pass = 0; pass < 2; ++pass) {
@@ -660,7 +760,7 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   }
   FH_CHECK_LAUNCH("key switch modup + column pass");
 
-  const int64_t groups = n / (kTile * kTilesPerBlock);
+  const int64_t groups = n / (kTile * kMacTilesPerBlock);
   for (int pass = 0; pass < 2; ++pass) {
     const int64_t j_begin = pass == 0 ? 0 : small_rows;
     const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
@@ -703,11 +803,9 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   md.has_base1 = base1.data != nullptr ? 1 : 0;
   md.gal_idx = galois_index_tables;
   md.gal_sign = galois_sign_tables;
-  hipLaunchKernelGGL(k_ks_moddown_add,
-                     dim3(static_cast<uint32_t>(ceil_div(n, kBlock)),
-                          static_cast<uint32_t>(ceil_div(lq, kModDownRowsPerThread)),
-                          static_cast<uint32_t>(2 * batch)),
-                     dim3(kBlock), 0, s, md);
+  const int64_t md_blocks = (n / kBlock) * ceil_div(lq, kModDownRowsPerThread) * 2 * batch;
+  FH_CHECK_ARG(md_blocks <= 0x7fffffff, "key switch grid too large");
+  hipLaunchKernelGGL(k_ks_moddown_add, dim3(static_cast<uint32_t>(md_blocks)), dim3(kBlock), 0, s, md);
   FH_CHECK_LAUNCH("key switch moddown + add");
   return kOk;
 }

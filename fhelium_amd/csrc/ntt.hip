@@ -151,17 +151,19 @@ __device__ __forceinline__ void fwd_tiles(const P& p, int log_n, int64_t* base, 
 // Inverse tile pass: stages 0..7 inside 256-element tiles.  Reads input
 // residues, writes intermediates.
 template <class P>
-__device__ __forceinline__ void inv_tiles(const P& p, int log_n, int64_t* base, const int64_t* tw,
-                                          const int64_t* twp, bool in_mid, int64_t* lds) {
+__device__ __forceinline__ void inv_tiles(const P& p, int log_n, int64_t* base, const int64_t* src,
+                                          const int64_t* tw, const int64_t* twp, bool in_mid,
+                                          int64_t* lds) {
   using V = typename P::V;
   const int m = threadIdx.x / 16;
   const int t = threadIdx.x % 16;
   const int r = blockIdx.x * kTilesPerBlock + m;
   int64_t* tile = base + static_cast<int64_t>(r) * kTile;
+  const int64_t* in_tile = src + static_cast<int64_t>(r) * kTile;
   int64_t* sh = lds + m * kTilePitch;
   const int n = 1 << log_n;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) sh[tile_pad(t + 16 * k)] = tile[t + 16 * k];
+  for (int k = 0; k < 16; ++k) sh[tile_pad(t + 16 * k)] = in_tile[t + 16 * k];
   __syncthreads();
   V v[16];
 #pragma unroll
@@ -264,6 +266,9 @@ struct NttArgs {
   int log_n;
   int form;             // forward: to-Montgomery flag; inverse: InvOut
   int in_mid;           // inverse only: input already in the working format
+  const int64_t* src;   // inverse only: input rows (null: in place)
+  int64_t src_batch_stride;
+  int64_t src_limb_stride;
 };
 
 struct RowRef {
@@ -271,6 +276,7 @@ struct RowRef {
   const NttRow* row;
   const int64_t* tw;
   const int64_t* twp;
+  const int64_t* src;   // input row of the first inverse pass
 };
 
 __device__ __forceinline__ RowRef row_ref(const NttArgs& a) {
@@ -279,8 +285,9 @@ __device__ __forceinline__ RowRef row_ref(const NttArgs& a) {
   const int64_t l = flat - b * a.limbs;
   const int64_t prow = a.row_start + l;
   const int64_t n = int64_t{1} << a.log_n;
-  return RowRef{a.data + b * a.batch_stride + l * a.limb_stride, a.rows + prow, a.tw + prow * n,
-                a.twp + prow * n};
+  int64_t* base = a.data + b * a.batch_stride + l * a.limb_stride;
+  const int64_t* src = a.src ? a.src + b * a.src_batch_stride + l * a.src_limb_stride : base;
+  return RowRef{base, a.rows + prow, a.tw + prow * n, a.twp + prow * n, src};
 }
 
 enum Mode : int { kExact = 0, kFast = 1 };
@@ -324,7 +331,8 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_inv_tiles(NttArgs a) {
   __shared__ int64_t lds[kTilesPerBlock * kTilePitch];
   const RowRef rr = row_ref(a);
-  FH_DISPATCH_PATH(MODE, *rr.row, (inv_tiles(p, a.log_n, rr.base, rr.tw, rr.twp, a.in_mid != 0, lds)));
+  FH_DISPATCH_PATH(MODE, *rr.row,
+                   (inv_tiles(p, a.log_n, rr.base, rr.src, rr.tw, rr.twp, a.in_mid != 0, lds)));
 }
 
 template <int S1, int MODE>
@@ -382,7 +390,8 @@ __global__ __launch_bounds__(kBlock) void k_stage(NttArgs a, int stage, int forw
 }
 
 // what: -1 forward to-Montgomery, -2 forward canonicalisation (fast mode
-// output), >= 0 inverse normalisation with that InvOut.
+// output), -3 copy of the inverse's input row (out-of-place inverse),
+// >= 0 inverse normalisation with that InvOut.
 __global__ __launch_bounds__(kBlock) void k_scale(NttArgs a, int what) {
   const int64_t n = int64_t{1} << a.log_n;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -390,7 +399,9 @@ __global__ __launch_bounds__(kBlock) void k_scale(NttArgs a, int what) {
   const RowRef rr = row_ref(a);
   const NttRow& row = *rr.row;
   int64_t* x = rr.base + i;
-  if (what == -2) {
+  if (what == -3) {
+    *x = rr.src[i];
+  } else if (what == -2) {
     const uint64_t v = static_cast<uint64_t>(*x);
     *x = static_cast<int64_t>(v >= row.q ? v - row.q : v);
   } else if (row.small) {
@@ -402,9 +413,10 @@ __global__ __launch_bounds__(kBlock) void k_scale(NttArgs a, int what) {
   }
 }
 
-// Rows per chunk for the two-pass schedule: chunk intermediates of
-// 128 rows x 512 KiB = 64 MiB stay resident in the Infinity Cache.
-int64_t g_chunk_rows = 128;
+// Rows per launch of the two-pass schedule (0: as many as the grid allows).
+// Measured on MI355X, splitting a transform into launches whose
+// intermediates would fit the Infinity Cache is slower than one launch.
+int64_t g_chunk_rows = 0;
 
 template <int MODE>
 void launch_columns(bool forward, int s1, dim3 grid, hipStream_t stream, const NttArgs& a) {
@@ -472,6 +484,10 @@ int launch(NttArgs a, int64_t total_rows, bool forward, int mode, hipStream_t st
     if (forward && a.form) {
       hipLaunchKernelGGL(k_scale, gel, dim3(kBlock), 0, stream, a, -1);
       FH_CHECK_LAUNCH("ntt to-montgomery scale");
+    }
+    if (!forward && a.src != nullptr) {
+      hipLaunchKernelGGL(k_scale, gel, dim3(kBlock), 0, stream, a, -3);
+      FH_CHECK_LAUNCH("ntt inverse input copy");
     }
     for (int s = 0; s < L; ++s) {
       hipLaunchKernelGGL(k_stage, gst, dim3(kBlock), 0, stream, a, s, forward ? 1 : 0);
@@ -636,6 +652,26 @@ FH_API int fh_ntt_inverse(int64_t* residues, int64_t batch, int64_t limbs, int64
   FH_CHECK_ARG(residues != nullptr, "NTT residues must be non-null");
   NttArgs a{residues, batch_stride, limb_stride, limbs, 0, static_cast<const NttRow*>(ntt_rows),
             twiddles, twiddles_shoup, row_start, static_cast<int>(log_n), output_form, 0};
+  return launch(a, batch * limbs, false, mode, static_cast<hipStream_t>(stream));
+}
+
+// Out-of-place inverse: reads `source` (same shape, its own strides), writes
+// `residues`; the source is not modified.
+FH_API int fh_ntt_inverse_from(int64_t* residues, const int64_t* source, int64_t batch, int64_t limbs,
+                               int64_t log_n, int64_t batch_stride, int64_t limb_stride,
+                               int64_t source_batch_stride, int64_t source_limb_stride,
+                               const void* ntt_rows, const int64_t* twiddles,
+                               const int64_t* twiddles_shoup, int64_t row_start, int32_t output_form,
+                               int32_t mode, void* stream) {
+  int st = check_ntt_common(batch, limbs, log_n, ntt_rows, twiddles, twiddles_shoup, row_start, mode);
+  if (st != kOk) return st;
+  FH_CHECK_ARG(output_form >= 0 && output_form <= 3, "inverse NTT output form %d is not 0..3",
+               output_form);
+  if (batch == 0 || limbs == 0) return kOk;
+  FH_CHECK_ARG(residues != nullptr && source != nullptr, "NTT residues must be non-null");
+  NttArgs a{residues, batch_stride, limb_stride, limbs, 0, static_cast<const NttRow*>(ntt_rows),
+            twiddles, twiddles_shoup, row_start, static_cast<int>(log_n), output_form, 0,
+            source, source_batch_stride, source_limb_stride};
   return launch(a, batch * limbs, false, mode, static_cast<hipStream_t>(stream));
 }
 

@@ -49,6 +49,8 @@ struct F64Exact {
   __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t*, int64_t idx) const {
     return Tw{__longlong_as_double(w[idx])};
   }
+  // twiddle from its table words (w, Shoup companion)
+  __device__ __forceinline__ Tw tw_make(int64_t w, int64_t) const { return Tw{__longlong_as_double(w)}; }
   __device__ __forceinline__ Tw konst(uint64_t c, uint64_t) const {
     return Tw{static_cast<double>(c)};
   }
@@ -99,6 +101,7 @@ struct F64Fast {
   __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t*, int64_t idx) const {
     return Tw{__longlong_as_double(w[idx])};
   }
+  __device__ __forceinline__ Tw tw_make(int64_t w, int64_t) const { return Tw{__longlong_as_double(w)}; }
   __device__ __forceinline__ Tw konst(uint64_t c, uint64_t) const {
     return Tw{static_cast<double>(c)};
   }
@@ -152,6 +155,9 @@ struct U64Path {
   __device__ __forceinline__ explicit U64Path(const NttRow& r) : q(r.q), twoq(r.twice_q), row(&r) {}
   __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t* wp, int64_t idx) const {
     return Tw{static_cast<uint64_t>(w[idx]), static_cast<uint64_t>(wp[idx])};
+  }
+  __device__ __forceinline__ Tw tw_make(int64_t w, int64_t wp) const {
+    return Tw{static_cast<uint64_t>(w), static_cast<uint64_t>(wp)};
   }
   __device__ __forceinline__ Tw konst(uint64_t c, uint64_t cp) const { return Tw{c, cp}; }
   __device__ __forceinline__ V in(int64_t raw) const { return static_cast<uint64_t>(raw); }

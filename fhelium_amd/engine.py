@@ -23,7 +23,8 @@ from fhelium_amd import kernels as K
 from fhelium_amd import rns
 from fhelium_amd.config import CkksConfig, Preset
 from fhelium_amd.core import (Ciphertext, CkksStateError, InvalidScaleError, KeySwitchKey,
-                              MaximumLevelError, RelinearizationKey, RotationKey, SecretKey)
+                              MaximumLevelError, Plaintext, RelinearizationKey, RotationKey,
+                              ScaleMismatchError, SecretKey)
 from fhelium_amd.keyswitch import HybridKeySwitcher
 from fhelium_amd.runtime import RnsRuntime
 
@@ -206,6 +207,97 @@ class CkksEngine:
         values = codec.crt_centered(np.moveaxis(res[..., sel, :], -2, 0), moduli)
         return codec.embed(values) / ct.scale
 
+    def encode(self, values, *, level: int = 0, scale: float | None = None,
+               polynomial_domain: str = "coefficient") -> Plaintext:
+        """Slots [*batch, <=S] -> operation-ready Montgomery plaintext in the
+        requested domain (ckks_plaintext_codec.py:181-316)."""
+
+        cfg = self.config
+        self.layout.check_level(level)
+        if polynomial_domain not in ("coefficient", "ntt"):
+            raise ValueError("polynomial_domain must be 'coefficient' or 'ntt'")
+        scale = cfg.default_scale if scale is None else float(scale)
+        if not (math.isfinite(scale) and scale > 0):
+            raise InvalidScaleError(f"plaintext scale must be positive and finite, got {scale}")
+        values = np.asarray(values)
+        coeffs = torch.from_numpy(codec.encode(values, cfg.N, scale))
+        data = self._lift_signed(coeffs, self.layout.prime_ids(level)).contiguous()
+        if polynomial_domain == "ntt":
+            self.runtime.ntt.forward_to_montgomery_(data, level)
+        else:
+            K.to_montgomery_(data, self.runtime.params(level))
+        return Plaintext(data=data, level=level, scale=scale, polynomial_domain=polynomial_domain)
+
+    def _check_plaintext(self, ct: Ciphertext, pt: Plaintext, domain: str) -> None:
+        if pt.level != ct.level:
+            raise ValueError(f"Plaintext level {pt.level} does not match ciphertext level {ct.level}.")
+        if pt.polynomial_domain != domain or pt.residue_representation != "montgomery":
+            raise CkksStateError(
+                f"plaintext must be {domain}-domain Montgomery residues for this operation")
+        if pt.modulus_basis != ct.modulus_basis or pt.data.shape[-2:] != ct.data.shape[-2:]:
+            raise ValueError("Plaintext and Ciphertext RNS layouts differ")
+        if pt.batch_shape and pt.batch_shape != ct.batch_shape:
+            raise ValueError("a batched plaintext must match the ciphertext batch shape")
+
+    def add_plaintext(self, ct: Ciphertext, plaintext: Plaintext, *, inplace: bool = False) -> Ciphertext:
+        """c0 + p at equal scale, coefficient/standard (ckks_engine.py:1964-2066)."""
+
+        ct.assert_state(polynomial_domain="coefficient", residue_representation="standard", components=2)
+        if plaintext.scale != ct.scale:
+            raise ScaleMismatchError(f"add_plaintext: ciphertext scale {ct.scale} != plaintext scale "
+                                     f"{plaintext.scale}")
+        self._check_plaintext(ct, plaintext, "coefficient")
+        params = self.runtime.params(ct.level, ct.includes_p)
+        if inplace:
+            K.add_prepared_plaintext_component_(ct.c0, plaintext.data, params)
+            return ct
+        c0 = K.add_prepared_plaintext_component(ct.c0, plaintext.data, params)
+        return Ciphertext(torch.stack([c0, ct.c1]), ct.level, ct.scale, "coefficient", "standard",
+                          ct.modulus_basis)
+
+    def multiply_plaintext(self, ct: Ciphertext, plaintext: Plaintext, *, inplace: bool = False) -> Ciphertext:
+        """c_j p in the NTT domain; scales multiply (ckks_engine.py:2077-2168)."""
+
+        ct.assert_state(polynomial_domain="ntt", residue_representation="montgomery", components=2)
+        self._check_plaintext(ct, plaintext, "ntt")
+        scale = ct.scale * plaintext.scale
+        if not (math.isfinite(scale) and scale > 0):
+            raise InvalidScaleError(f"multiply_plaintext result scale {scale} is not positive and finite")
+        params = self.runtime.params(ct.level, ct.includes_p)
+        pt = plaintext.data
+        if pt.dim() < ct.data.dim() - 1:
+            pt = pt.expand(ct.data.shape[1:])
+        out = torch.empty_like(ct.data)
+        out[0] = K.montgomery_mul(ct.c0, pt, params)
+        out[1] = K.montgomery_mul(ct.c1, pt, params)
+        result = Ciphertext(out, ct.level, scale, "ntt", "montgomery", ct.modulus_basis)
+        if inplace:
+            ct.data, ct.scale = result.data, result.scale
+            return ct
+        return result
+
+    def add(self, lhs: Ciphertext, rhs: Ciphertext, *, inplace: bool = False) -> Ciphertext:
+        """Component-wise canonical sum at equal scale and state (ckks_engine.py:1708-1745, 3352-3377)."""
+
+        self._same_layout(lhs, rhs)
+        if lhs.scale != rhs.scale:
+            raise ScaleMismatchError(f"add: scales differ ({lhs.scale} != {rhs.scale})")
+        if lhs.component_count != rhs.component_count:
+            raise CkksStateError("add: component counts differ")
+        params = self.runtime.params(lhs.level, lhs.includes_p)
+        result = lhs if inplace else lhs.clone()
+        flat = result.data.view(-1, result.data.size(-2), result.data.size(-1))
+        K.add_canonical_(flat, rhs.data.reshape(flat.shape), params)
+        return result
+
+    def ntt_domain_to_coefficient_domain(self, ct: Ciphertext) -> Ciphertext:
+        """NTT/Montgomery -> coefficient/standard canonical (ckks_engine.py:931-985)."""
+
+        ct.assert_state(polynomial_domain="ntt", residue_representation="montgomery")
+        rt = self.runtime
+        data = rt.ntt.inverse_to_standard(ct.data, rt.row_start(ct.level))
+        return Ciphertext(data, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
+
     # ------------------------------------------------------------------
     # The hot path
     # ------------------------------------------------------------------
@@ -245,8 +337,7 @@ class CkksEngine:
         if key is None:
             key = self.generate_relinearization_key()
         rt = self.runtime
-        d = ct.data.clone()
-        rt.ntt.inverse_to_standard_(d, rt.row_start(ct.level))
+        d = rt.ntt.inverse_to_standard(ct.data, rt.row_start(ct.level))
         if self.fused_keyswitch:
             out = self.key_switcher.key_switch_fused(d[2], key, ct.level, base0=d[0], base1=d[1])
         else:

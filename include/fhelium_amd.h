@@ -73,6 +73,15 @@ int fh_ntt_inverse(int64_t* residues, int64_t batch, int64_t limbs, int64_t log_
                    int64_t batch_stride, int64_t limb_stride, const void* ntt_rows,
                    const int64_t* twiddles, const int64_t* twiddles_shoup, int64_t row_start,
                    int32_t output_form, int32_t mode, void* stream);
+/* fh_ntt_inverse reading `source` (same [batch, limb, N] shape, its own
+ * strides) and writing `residues`; `source` is left unchanged.  Replaces the
+ * clone + in-place inverse of relinearize (ckks_engine.py:1608-1675). */
+int fh_ntt_inverse_from(int64_t* residues, const int64_t* source, int64_t batch, int64_t limbs,
+                        int64_t log_n, int64_t batch_stride, int64_t limb_stride,
+                        int64_t source_batch_stride, int64_t source_limb_stride,
+                        const void* ntt_rows, const int64_t* twiddles,
+                        const int64_t* twiddles_shoup, int64_t row_start, int32_t output_form,
+                        int32_t mode, void* stream);
 /* One stage of forward/inverse_ntt_*_indexed_ (indexed_ntt_cpu.cpp:96-170,
  * 206-258) with explicit even/odd index rows and the stage's expanded
  * Montgomery twiddles; the caller validates the schedule first. */

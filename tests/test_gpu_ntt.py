@@ -94,7 +94,7 @@ def test_row_start_and_strided_batch(chunk_rows):
     back = O.ntt_inverse(want, np.ascontiguousarray(inv[sub]), np.ascontiguousarray(params[:, sub]), 2)
     np.testing.assert_array_equal(H.to_np(view), back)
     np.testing.assert_array_equal(back, x)  # to-Montgomery then standard inverse is the identity
-    nat.call("fh_ntt_set_chunk_rows", 128)
+    nat.call("fh_ntt_set_chunk_rows", 0)
 
 
 @pytest.mark.parametrize("mode", ["exact", "fast"])
@@ -143,3 +143,22 @@ def test_ring_kernels_full_workload(mode):
         nb = O.ntt_forward(xb[None], f35, p35, to_montgomery=True)
         want = O.ntt_inverse(O.binary("mul", na, nb, p35), i35, p35, 2)[0]
         np.testing.assert_array_equal(H.to_np(prod[i, c]), want)
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("log_n", [9, 13, 16])
+def test_inverse_out_of_place(log_n, mode):
+    """inverse_to_standard(x) reads a strided level-3 view and leaves it
+    untouched; the result equals the in-place inverse bit for bit."""
+    cfg, params, _, inv, be = _backend(log_n, mode)
+    rng = np.random.default_rng(3000 + log_n)
+    q = H.moduli_array(cfg, 3, 32)
+    x = H.random_residues(rng, 3, q, cfg.N, lazy=True)
+    big = torch.zeros((3, 40, cfg.N), dtype=torch.int64, device="cuda")
+    big[:, 5:37] = H.to_dev(x)
+    view = big[:, 5:37]
+    before = big.clone()
+    out = be.inverse_to_standard(view, 3)
+    assert torch.equal(big, before)
+    want = O.ntt_inverse(x, inv[3:35], np.ascontiguousarray(params[:, 3:35]), 2)
+    np.testing.assert_array_equal(H.to_np(out), want)
