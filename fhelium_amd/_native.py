@@ -53,6 +53,8 @@ _ptr = ctypes.c_void_p
 SIGNATURES: dict[str, list] = {
     "fh_ntt_forward": [_ptr, _i64, _i64, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i32, _i32, _ptr],
     "fh_ntt_inverse": [_ptr, _i64, _i64, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i32, _i32, _ptr],
+    "fh_ntt_forward_from": [_ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64,
+                            _i32, _i32, _ptr],
     "fh_ntt_inverse_from": [_ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64,
                             _i32, _i32, _ptr],
     "fh_ntt_set_chunk_rows": [_i64],

@@ -216,19 +216,19 @@ __device__ __forceinline__ uint64_t shoup_mulmod(uint64_t w, uint64_t wp,
   return r >= q ? r - q : r;
 }
 
-// Exact u64 <-> f64 conversions for values below 2^53.
+// Exact u64 <-> f64 conversions for integers below 2^52 (every value the
+// binary64 paths convert is below 2^47): 2^52 + v has the bit pattern
+// 0x4330000000000000 | v, so each direction is one integer and one binary64
+// operation.
+constexpr double kTwo52 = 4503599627370496.0;
+constexpr uint64_t kTwo52Bits = 0x4330000000000000ull;
+
 __device__ __forceinline__ double u64_to_f64(uint64_t v) {
-  const uint32_t lo = static_cast<uint32_t>(v);
-  const uint32_t hi = static_cast<uint32_t>(v >> 32);
-  return __builtin_fma(static_cast<double>(hi), 4294967296.0,
-                       static_cast<double>(lo));
+  return __longlong_as_double(static_cast<long long>(v | kTwo52Bits)) - kTwo52;
 }
 
 __device__ __forceinline__ uint64_t f64_to_u64(double v) {
-  const uint32_t hi = static_cast<uint32_t>(v * 2.3283064365386963e-10);
-  const double rest = __builtin_fma(-static_cast<double>(hi), 4294967296.0, v);
-  const uint32_t lo = static_cast<uint32_t>(rest);
-  return (static_cast<uint64_t>(hi) << 32) | lo;
+  return static_cast<uint64_t>(__double_as_longlong(v + kTwo52)) ^ kTwo52Bits;
 }
 
 // Per-row constants of the gfx950 NTT backend, one 128-byte record per QP

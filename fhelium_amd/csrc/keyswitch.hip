@@ -91,6 +91,8 @@ struct KsArgs {
   const NttRow* rows;       // global QP NTT records
   const int64_t* fwd_w;     // global forward twiddles [QP0, N]
   const int64_t* fwd_wp;
+  const int64_t* inv_w;     // global inverse twiddles [QP0, N]
+  const int64_t* inv_wp;
   const int64_t* key;       // shared key [digit, 2, QP0, N] or null
   const int64_t* key_bases; // per-entry key addresses [B] or null
   int64_t key_sd, key_sc, key_sr;   // key strides (digit, component, row)
@@ -101,12 +103,14 @@ struct KsArgs {
 // Block i runs on XCD i % 8; give each XCD one contiguous range of work
 // items (batch fastest) so consecutive items on an XCD share key tiles in
 // its L2.  The map is a bijection of [0, total).
-__device__ __forceinline__ int64_t xcd_remap(int64_t i, int64_t total) {
-  constexpr int64_t kXcds = 8;
-  const int64_t x = i % kXcds;
-  const int64_t s = i / kXcds;
-  const int64_t q = total / kXcds;
-  const int64_t rem = total % kXcds;
+// Block-index arithmetic is 32-bit (grids are below 2^31 blocks): 64-bit
+// division expands to a long instruction sequence on the GPU.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t total) {
+  constexpr uint32_t kXcds = 8;
+  const uint32_t x = i % kXcds;
+  const uint32_t s = i / kXcds;
+  const uint32_t q = total / kXcds;
+  const uint32_t rem = total % kXcds;
   return x * q + (x < rem ? x : rem) + s;
 }
 
@@ -161,7 +165,10 @@ __device__ __forceinline__ void garner(const KsDigit& d, int64_t* dg) {
   for (int step = 0; step < kMaxDigitRows - 1; ++step) {
     const int row = step + 1;
     if (row >= rows) break;
-    const int64_t digit = mont_mul_split(src[row] - dg[row], global(d.normalizers)[step], digit_mont(d, row));
+    // multiply_split reduces both operands; the normalizer is canonical
+    // already, so only the difference needs reducing.
+    const Mont mr = digit_mont(d, row);
+    const int64_t digit = mont_mul(canonical_operand(src[row] - dg[row], mr.q), global(d.normalizers)[step], mr);
     dg[row] = digit;
 #pragma unroll
     for (int t = 0; t < kMaxDigitRows; ++t) {
@@ -178,11 +185,12 @@ __global__ __launch_bounds__(kBlock) void k_ks_decompose(KsArgs a) {
   const int64_t n = int64_t{1} << a.log_n;
   // One XCD walks all coefficient chunks of an (entry, digit): a Galois
   // gather then touches each source row in one L2 only.
-  const int64_t chunks = n / kBlock;
-  const int64_t w = xcd_remap(blockIdx.x, chunks * a.dnum * a.batch);
-  const int64_t i = (w % chunks) * kBlock + threadIdx.x;
-  const KsDigit& d = a.dig[(w / chunks) % a.dnum];
-  const int64_t b = w / (chunks * a.dnum);
+  const uint32_t chunks = static_cast<uint32_t>(n / kBlock);
+  const uint32_t dnum = static_cast<uint32_t>(a.dnum);
+  const uint32_t w = xcd_remap(blockIdx.x, chunks * dnum * static_cast<uint32_t>(a.batch));
+  const int64_t i = static_cast<int64_t>(w % chunks) * kBlock + threadIdx.x;
+  const KsDigit& d = a.dig[(w / chunks) % dnum];
+  const int64_t b = w / (chunks * dnum);
   const int rows = static_cast<int>(d.rows);
   int64_t dg[kMaxDigitRows];
 #pragma unroll
@@ -325,8 +333,9 @@ template <int S1, int MODE, bool kSmall>
 __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j_begin) {
   __shared__ int64_t lds[(1 << S1) * (kBlock / ((1 << S1) / 16))];
   const int64_t j = j_begin + blockIdx.y;
-  const int64_t b = blockIdx.z / a.dnum;
-  const int64_t di = blockIdx.z % a.dnum;
+  const uint32_t dnum = static_cast<uint32_t>(a.dnum);
+  const int64_t b = blockIdx.z / dnum;
+  const int64_t di = blockIdx.z % dnum;
   const KsDigit& d = a.dig[di];
   const NttRow& row = a.rows[a.level + j];
   if constexpr (kSmall) {
@@ -350,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j
 // so the sum is (sum k y) R mod q -- the inverse NTT removes that R
 // (kOutStdMac).  |acc| stays below 6q over at most 10 digits.
 __device__ __forceinline__ void mac_f64(double& acc, double y, int64_t key, double q, double qinv) {
-  acc += f64_mulmod_signed(static_cast<double>(key), y, q, qinv);
+  acc += f64_mulmod_signed(u64_to_f64(static_cast<uint64_t>(key)), y, q, qinv);
 }
 
 
@@ -493,17 +502,83 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       }
     }
   }
+  // The first 8 stages of the accumulators' inverse NTT act inside this
+  // tile: run them here (fast arithmetic, as the rest of the inverse) and
+  // write the working-format result; only the column pass remains.
+  __syncthreads();
+  {
+    const int64_t* w_row = a.inv_w + grow * n;
+    const int64_t* wp_row = a.inv_wp + grow * n;
+    for (int idx = t; idx < kTile - 1; idx += kMacThreads) {
+      // stage st owns slots [256 - 2^(8-st), 256 - 2^(7-st)); group g = e >> (st+1)
+      const int st = 7 - (31 - __builtin_clz(kTile - 1 - idx));
+      const int g = idx - (kTile - (kTile >> st));
+      const int64_t gi = (n >> (st + 1)) + (static_cast<int64_t>(r) << (7 - st)) + g;
+      tww[idx] = w_row[gi];
+      if constexpr (!kF64) twp[idx] = wp_row[gi];
+    }
+  }
+  __syncthreads();
+  auto inv_slot = [](int st, int e) { return (kTile - (kTile >> st)) + (e >> (st + 1)); };
+  auto inverse_tile = [&](const auto& pi, auto (&v)[8], int64_t* out) {
+    // Round 1 (slots = bits 2..0 of the element): inverse stages 0..2.
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+      const int half = 1 << st;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        const int slot = inv_slot(st, mac_e3(t, k));
+        pi.inv(v[k], v[k + half], pi.tw_make(tww[slot], kF64 ? 0 : twp[slot]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e3(t, k))] = pi.mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = pi.unmid(sh[tile_pad(mac_e2(t, k))]);
+    // Round 2 (slots = bits 4..2): stages 3, 4.
+#pragma unroll
+    for (int st = 3; st < 5; ++st) {
+      const int half = 1 << (st - 2);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        const int slot = inv_slot(st, mac_e2(t, k));
+        pi.inv(v[k], v[k + half], pi.tw_make(tww[slot], kF64 ? 0 : twp[slot]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = pi.mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = pi.unmid(sh[tile_pad(mac_e1(t, k))]);
+    // Round 3 (slots = bits 7..5): stages 5..7.
+#pragma unroll
+    for (int st = 5; st < 8; ++st) {
+      const int half = 1 << (st - 5);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        const int slot = inv_slot(st, mac_e1(t, k));
+        pi.inv(v[k], v[k + half], pi.tw_make(tww[slot], kF64 ? 0 : twp[slot]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[mac_e1(t, k)] = pi.mid(v[k]);
+    // the exchange buffer is reused by the next accumulator
+    __syncthreads();
+  };
   int64_t* out0 = a.acc + ((0 * a.batch + b) * a.lqp + j) * n + tile_off;
   int64_t* out1 = a.acc + ((1 * a.batch + b) * a.lqp + j) * n + tile_off;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    if constexpr (kF64) {
-      out0[mac_e3(t, k)] = __double_as_longlong(facc0[k]);
-      out1[mac_e3(t, k)] = __double_as_longlong(facc1[k]);
-    } else {
-      out0[mac_e3(t, k)] = static_cast<int64_t>(uacc0[k]);
-      out1[mac_e3(t, k)] = static_cast<int64_t>(uacc1[k]);
-    }
+  if constexpr (kF64) {
+    const F64Fast pi(row);
+    inverse_tile(pi, facc0, out0);
+    inverse_tile(pi, facc1, out1);
+  } else {
+    const U64Path<false> pi(row);
+    inverse_tile(pi, uacc0, out0);
+    inverse_tile(pi, uacc1, out1);
   }
 }
 
@@ -521,11 +596,12 @@ template <int MODE, bool kSmall>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
                                                          int64_t j_count) {
   __shared__ int64_t lds[mac_lds_words<kSmall>()];
-  const int64_t total = groups * j_count * a.batch;
-  const int64_t w = xcd_remap(blockIdx.x, total);
-  const int64_t b = w % a.batch;
-  const int64_t j = j_begin + (w / a.batch) % j_count;
-  const int64_t g = w / (a.batch * j_count);
+  const uint32_t batch = static_cast<uint32_t>(a.batch);
+  const uint32_t jc = static_cast<uint32_t>(j_count);
+  const uint32_t w = xcd_remap(blockIdx.x, static_cast<uint32_t>(groups) * jc * batch);
+  const int64_t b = w % batch;
+  const int64_t j = j_begin + (w / batch) % jc;
+  const int64_t g = w / (batch * jc);
   const NttRow& row = a.rows[a.level + j];
   if constexpr (kSmall) {
     if constexpr (MODE == 1) {
@@ -563,14 +639,15 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   const int64_t n = int64_t{1} << a.log_n;
   // XCD-contiguous work items: all coefficient chunks of one (accumulator,
   // entry, row group) run on one XCD (see k_ks_decompose).
-  const int64_t chunks = n / kBlock;
-  const int64_t groups = (a.lq + kRows - 1) / kRows;
-  const int64_t w = xcd_remap(blockIdx.x, chunks * groups * 2 * a.batch);
-  const int64_t i = (w % chunks) * kBlock + threadIdx.x;
+  const uint32_t chunks = static_cast<uint32_t>(n / kBlock);
+  const uint32_t groups = static_cast<uint32_t>((a.lq + kRows - 1) / kRows);
+  const uint32_t batch = static_cast<uint32_t>(a.batch);
+  const uint32_t w = xcd_remap(blockIdx.x, chunks * groups * 2 * batch);
+  const int64_t i = static_cast<int64_t>(w % chunks) * kBlock + threadIdx.x;
   const int64_t group = (w / chunks) % groups;
-  const int64_t cb = w / (chunks * groups);
-  const int64_t c = cb / a.batch;
-  const int64_t b = cb % a.batch;
+  const uint32_t cb = w / (chunks * groups);
+  const int64_t c = cb / batch;
+  const int64_t b = cb % batch;
   const int p_count = static_cast<int>(a.lqp - a.lq);
   const int64_t* x = a.acc + (c * a.batch + b) * a.lqp * n + i;
   const View3& base = c == 0 ? a.base0 : a.base1;
@@ -715,6 +792,8 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   a.rows = static_cast<const NttRow*>(ntt_rows);
   a.fwd_w = forward_twiddles;
   a.fwd_wp = forward_twiddles_shoup;
+  a.inv_w = inverse_twiddles;
+  a.inv_wp = inverse_twiddles_shoup;
   a.key = key;
   a.key_bases = key_bases;
   a.key_sd = key_stride_digit;
@@ -779,9 +858,10 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   FH_CHECK_LAUNCH("key switch tile pass + MAC");
 
   // Both accumulators as one [2B, Lqp, N] inverse transform.
-  int st = ntt::inverse_internal(workspace_acc, 2 * batch, lqp, static_cast<int>(log_n), lqp * n, n,
-                                 static_cast<const NttRow*>(ntt_rows), inverse_twiddles,
-                                 inverse_twiddles_shoup, level, kOutStdMac, true, s);
+  // The tile stages of this inverse ran inside the MAC kernel.
+  int st = ntt::inverse_columns_internal(workspace_acc, 2 * batch, lqp, static_cast<int>(log_n), lqp * n, n,
+                                         static_cast<const NttRow*>(ntt_rows), inverse_twiddles,
+                                         inverse_twiddles_shoup, level, kOutStdMac, s);
   if (st != kOk) return st;
 
   ModDownArgs md;

@@ -50,8 +50,9 @@ using namespace ::fh::ntt;
 // (element 256*i + c).  Block: C = 256/T columns x T = 2^{S1-4} threads.
 // Reads input residues, writes intermediates.
 template <int S1, class P>
-__device__ __forceinline__ void fwd_columns(const P& p, int64_t* base, const int64_t* tw,
-                                            const int64_t* twp, bool to_mont, int64_t* lds) {
+__device__ __forceinline__ void fwd_columns(const P& p, int64_t* base, const int64_t* src,
+                                            const int64_t* tw, const int64_t* twp, bool to_mont,
+                                            int64_t* lds) {
   constexpr int R1 = 1 << S1;
   constexpr int T = R1 / 16;
   constexpr int C = kBlock / T;
@@ -61,7 +62,7 @@ __device__ __forceinline__ void fwd_columns(const P& p, int64_t* base, const int
   const int col = blockIdx.x * C + c;
   V v[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = p.in(base[static_cast<int64_t>(t + T * k) * kTile + col]);
+  for (int k = 0; k < 16; ++k) v[k] = p.in(src[static_cast<int64_t>(t + T * k) * kTile + col]);
   if (to_mont) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = p.to_mont(v[k]);
@@ -266,7 +267,7 @@ struct NttArgs {
   int log_n;
   int form;             // forward: to-Montgomery flag; inverse: InvOut
   int in_mid;           // inverse only: input already in the working format
-  const int64_t* src;   // inverse only: input rows (null: in place)
+  const int64_t* src;   // input rows of the first pass (null: in place)
   int64_t src_batch_stride;
   int64_t src_limb_stride;
 };
@@ -276,13 +277,15 @@ struct RowRef {
   const NttRow* row;
   const int64_t* tw;
   const int64_t* twp;
-  const int64_t* src;   // input row of the first inverse pass
+  const int64_t* src;   // input row of the first pass
 };
 
 __device__ __forceinline__ RowRef row_ref(const NttArgs& a) {
-  const int64_t flat = a.row_offset + blockIdx.y;
-  const int64_t b = flat / a.limbs;
-  const int64_t l = flat - b * a.limbs;
+  // 32-bit division: flat rows stay below 2^31 (64-bit division is a long
+  // instruction sequence on the GPU)
+  const uint32_t flat = static_cast<uint32_t>(a.row_offset) + blockIdx.y;
+  const int64_t b = flat / static_cast<uint32_t>(a.limbs);
+  const int64_t l = flat - static_cast<uint32_t>(b) * static_cast<uint32_t>(a.limbs);
   const int64_t prow = a.row_start + l;
   const int64_t n = int64_t{1} << a.log_n;
   int64_t* base = a.data + b * a.batch_stride + l * a.limb_stride;
@@ -317,7 +320,7 @@ template <int S1, int MODE>
 __global__ __launch_bounds__(kBlock) void k_fwd_columns(NttArgs a) {
   __shared__ int64_t lds[(1 << S1) * (kBlock / ((1 << S1) / 16))];
   const RowRef rr = row_ref(a);
-  FH_DISPATCH_PATH(MODE, *rr.row, (fwd_columns<S1>(p, rr.base, rr.tw, rr.twp, a.form != 0, lds)));
+  FH_DISPATCH_PATH(MODE, *rr.row, (fwd_columns<S1>(p, rr.base, rr.src, rr.tw, rr.twp, a.form != 0, lds)));
 }
 
 template <int MODE>
@@ -390,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(NttArgs a, int stage, int forw
 }
 
 // what: -1 forward to-Montgomery, -2 forward canonicalisation (fast mode
-// output), -3 copy of the inverse's input row (out-of-place inverse),
+// output), -3 copy of the input row (out-of-place transforms),
 // >= 0 inverse normalisation with that InvOut.
 __global__ __launch_bounds__(kBlock) void k_scale(NttArgs a, int what) {
   const int64_t n = int64_t{1} << a.log_n;
@@ -481,13 +484,13 @@ int launch(NttArgs a, int64_t total_rows, bool forward, int mode, hipStream_t st
     a.row_offset = start;
     const dim3 gel(static_cast<uint32_t>(ceil_div(n, kBlock)), static_cast<uint32_t>(rows));
     const dim3 gst(static_cast<uint32_t>(ceil_div(n / 2, kBlock)), static_cast<uint32_t>(rows));
+    if (a.src != nullptr) {
+      hipLaunchKernelGGL(k_scale, gel, dim3(kBlock), 0, stream, a, -3);
+      FH_CHECK_LAUNCH("ntt input copy");
+    }
     if (forward && a.form) {
       hipLaunchKernelGGL(k_scale, gel, dim3(kBlock), 0, stream, a, -1);
       FH_CHECK_LAUNCH("ntt to-montgomery scale");
-    }
-    if (!forward && a.src != nullptr) {
-      hipLaunchKernelGGL(k_scale, gel, dim3(kBlock), 0, stream, a, -3);
-      FH_CHECK_LAUNCH("ntt inverse input copy");
     }
     for (int s = 0; s < L; ++s) {
       hipLaunchKernelGGL(k_stage, gst, dim3(kBlock), 0, stream, a, s, forward ? 1 : 0);
@@ -622,6 +625,25 @@ int inverse_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n, int
   return launch(a, batch * limbs, false, kFast, stream);
 }
 
+int inverse_columns_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n,
+                             int64_t batch_stride, int64_t limb_stride, const NttRow* rows,
+                             const int64_t* tw, const int64_t* twp, int64_t row_start,
+                             int output_form, hipStream_t stream) {
+  FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "column-only inverse needs logN 12..16");
+  NttArgs a{data, batch_stride, limb_stride, limbs, 0, rows, tw, twp, row_start, log_n, output_form, 1};
+  const int s1 = log_n - kTileLog;
+  const int T = (1 << s1) / 16;
+  const int C = kBlock / T;
+  const int64_t total_rows = batch * limbs;
+  for (int64_t start = 0; start < total_rows; start += kMaxGridY) {
+    const int64_t rows_now = total_rows - start < kMaxGridY ? total_rows - start : kMaxGridY;
+    a.row_offset = start;
+    launch_columns<kFast>(false, s1, dim3(kTile / C, static_cast<uint32_t>(rows_now)), stream, a);
+    FH_CHECK_LAUNCH("ntt inverse column pass");
+  }
+  return kOk;
+}
+
 }  // namespace ntt
 }  // namespace fh
 
@@ -655,8 +677,24 @@ FH_API int fh_ntt_inverse(int64_t* residues, int64_t batch, int64_t limbs, int64
   return launch(a, batch * limbs, false, mode, static_cast<hipStream_t>(stream));
 }
 
-// Out-of-place inverse: reads `source` (same shape, its own strides), writes
-// `residues`; the source is not modified.
+// Out-of-place transforms: read `source` (same shape, its own strides),
+// write `residues`; the source is not modified.
+FH_API int fh_ntt_forward_from(int64_t* residues, const int64_t* source, int64_t batch, int64_t limbs,
+                               int64_t log_n, int64_t batch_stride, int64_t limb_stride,
+                               int64_t source_batch_stride, int64_t source_limb_stride,
+                               const void* ntt_rows, const int64_t* twiddles,
+                               const int64_t* twiddles_shoup, int64_t row_start, int32_t to_montgomery,
+                               int32_t mode, void* stream) {
+  int st = check_ntt_common(batch, limbs, log_n, ntt_rows, twiddles, twiddles_shoup, row_start, mode);
+  if (st != kOk) return st;
+  if (batch == 0 || limbs == 0) return kOk;
+  FH_CHECK_ARG(residues != nullptr && source != nullptr, "NTT residues must be non-null");
+  NttArgs a{residues, batch_stride, limb_stride, limbs, 0, static_cast<const NttRow*>(ntt_rows),
+            twiddles, twiddles_shoup, row_start, static_cast<int>(log_n), to_montgomery ? 1 : 0, 0,
+            source, source_batch_stride, source_limb_stride};
+  return launch(a, batch * limbs, true, mode, static_cast<hipStream_t>(stream));
+}
+
 FH_API int fh_ntt_inverse_from(int64_t* residues, const int64_t* source, int64_t batch, int64_t limbs,
                                int64_t log_n, int64_t batch_stride, int64_t limb_stride,
                                int64_t source_batch_stride, int64_t source_limb_stride,

@@ -204,6 +204,12 @@ struct U64Path {
 int inverse_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n, int64_t batch_stride,
                      int64_t limb_stride, const NttRow* rows, const int64_t* tw, const int64_t* twp,
                      int64_t row_start, int output_form, bool in_mid, hipStream_t stream);
+// Only the column pass of that inverse (logN 12..16): for rows whose tile
+// stages were already applied, holding working-format values.
+int inverse_columns_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n,
+                             int64_t batch_stride, int64_t limb_stride, const NttRow* rows,
+                             const int64_t* tw, const int64_t* twp, int64_t row_start,
+                             int output_form, hipStream_t stream);
 
 }  // namespace ntt
 }  // namespace fh

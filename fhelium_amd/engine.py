@@ -305,11 +305,8 @@ class CkksEngine:
         """Coefficient/standard -> NTT/Montgomery (ckks_engine.py:876-903, 3094-3109)."""
 
         ct.assert_state(polynomial_domain="coefficient", residue_representation="standard")
-        out = ct.clone()
-        self.runtime.ntt.forward_to_montgomery_(out.data, self.runtime.row_start(ct.level))
-        out.polynomial_domain = "ntt"
-        out.residue_representation = "montgomery"
-        return out
+        data = self.runtime.ntt.forward_to_montgomery(ct.data, self.runtime.row_start(ct.level))
+        return Ciphertext(data, ct.level, ct.scale, "ntt", "montgomery", ct.modulus_basis, ct.context_id)
 
     def _same_layout(self, a: Ciphertext, b: Ciphertext) -> None:
         if a.level != b.level or a.data.shape[1:] != b.data.shape[1:] or a.modulus_basis != b.modulus_basis:

@@ -118,9 +118,7 @@ class Gfx950NttBackend:
         self._run("fh_ntt_forward", operand, parameter_row_start, 1)
 
     def forward_to_montgomery(self, operand: torch.Tensor, parameter_row_start: int) -> torch.Tensor:
-        out = operand.clone(memory_format=torch.contiguous_format)
-        self.forward_to_montgomery_(out, parameter_row_start)
-        return out
+        return self._run_from("fh_ntt_forward_from", operand, parameter_row_start, 1)
 
     def inverse_montgomery_(self, operand: torch.Tensor, parameter_row_start: int) -> None:
         self._run("fh_ntt_inverse", operand, parameter_row_start, OUT_MONTGOMERY)
@@ -134,32 +132,37 @@ class Gfx950NttBackend:
     def inverse_to_centered_(self, operand: torch.Tensor, parameter_row_start: int) -> None:
         self._run("fh_ntt_inverse", operand, parameter_row_start, OUT_CENTERED)
 
-    def inverse_to_standard(self, operand: torch.Tensor, parameter_row_start: int) -> torch.Tensor:
-        """Out-of-place inverse to canonical standard residues: a new
-        contiguous tensor, ``operand`` unchanged (one pass reads it, so no
-        clone is needed)."""
+    def _run_from(self, fn: str, operand: torch.Tensor, row_start: int, form: int) -> torch.Tensor:
+        """Out-of-place transform into a new contiguous tensor; ``operand``
+        is left unchanged (the first pass reads it, so no clone is made)."""
 
         nat.require_device(operand)
         t = self.tables
         if operand.dtype != torch.int64 or operand.size(-1) != (1 << t.log_n):
             raise ValueError("operand must be int64 [..., limb, N] for this ring")
-        if parameter_row_start < 0 or parameter_row_start + operand.size(-2) > t.rows.size(0):
+        if row_start < 0 or row_start + operand.size(-2) > t.rows.size(0):
             raise ValueError("NTT operand rows exceed the prepared canonical parameter range")
         src = nat.batch_view(operand, "residues")
         if src.stride(2) != 1:
             raise ValueError("NTT operands need a unit coefficient stride")
         out = torch.empty(operand.shape, dtype=operand.dtype, device=operand.device)
         dst = out.view(src.shape)
+        fwd = fn == "fh_ntt_forward_from"
         rec = self.recorder
         if rec is not None:
             start = torch.cuda.Event(enable_timing=True)
             end = torch.cuda.Event(enable_timing=True)
             start.record()
-        nat.call("fh_ntt_inverse_from", dst.data_ptr(), src.data_ptr(), src.size(0), src.size(1), t.log_n,
+        nat.call(fn, dst.data_ptr(), src.data_ptr(), src.size(0), src.size(1), t.log_n,
                  dst.stride(0), dst.stride(1), src.stride(0), src.stride(1), t.rows.data_ptr(),
-                 t.inv.data_ptr(), t.inv_shoup.data_ptr(), parameter_row_start, OUT_STANDARD, self._mode,
-                 nat.stream_ptr(operand.device))
+                 (t.fwd if fwd else t.inv).data_ptr(), (t.fwd_shoup if fwd else t.inv_shoup).data_ptr(),
+                 row_start, form, self._mode, nat.stream_ptr(operand.device))
         if rec is not None:
             end.record()
             rec.append((start, end, 2 * src.size(0) * src.size(1) * src.size(2) * 8))
         return out
+
+    def inverse_to_standard(self, operand: torch.Tensor, parameter_row_start: int) -> torch.Tensor:
+        """Out-of-place inverse to canonical standard residues."""
+
+        return self._run_from("fh_ntt_inverse_from", operand, parameter_row_start, OUT_STANDARD)

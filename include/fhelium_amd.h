@@ -73,6 +73,16 @@ int fh_ntt_inverse(int64_t* residues, int64_t batch, int64_t limbs, int64_t log_
                    int64_t batch_stride, int64_t limb_stride, const void* ntt_rows,
                    const int64_t* twiddles, const int64_t* twiddles_shoup, int64_t row_start,
                    int32_t output_form, int32_t mode, void* stream);
+/* fh_ntt_forward reading `source` (same [batch, limb, N] shape, its own
+ * strides) and writing `residues`; `source` is left unchanged.  Replaces the
+ * clone + in-place transform of coefficient_domain_to_ntt_domain
+ * (ckks_engine.py:876-903). */
+int fh_ntt_forward_from(int64_t* residues, const int64_t* source, int64_t batch, int64_t limbs,
+                        int64_t log_n, int64_t batch_stride, int64_t limb_stride,
+                        int64_t source_batch_stride, int64_t source_limb_stride,
+                        const void* ntt_rows, const int64_t* twiddles,
+                        const int64_t* twiddles_shoup, int64_t row_start, int32_t to_montgomery,
+                        int32_t mode, void* stream);
 /* fh_ntt_inverse reading `source` (same [batch, limb, N] shape, its own
  * strides) and writing `residues`; `source` is left unchanged.  Replaces the
  * clone + in-place inverse of relinearize (ckks_engine.py:1608-1675). */

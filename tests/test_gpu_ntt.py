@@ -162,3 +162,23 @@ def test_inverse_out_of_place(log_n, mode):
     assert torch.equal(big, before)
     want = O.ntt_inverse(x, inv[3:35], np.ascontiguousarray(params[:, 3:35]), 2)
     np.testing.assert_array_equal(H.to_np(out), want)
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("log_n", [9, 13, 16])
+def test_forward_out_of_place(log_n, mode):
+    """forward_to_montgomery(x) reads a strided level-3 view and leaves it
+    untouched; the result equals the oracle (exact) or its canonical form."""
+    cfg, params, fwd, _, be = _backend(log_n, mode)
+    rng = np.random.default_rng(4000 + log_n)
+    q = H.moduli_array(cfg, 3, 32)
+    x = H.random_residues(rng, 3, q, cfg.N)
+    big = torch.zeros((3, 40, cfg.N), dtype=torch.int64, device="cuda")
+    big[:, 5:37] = H.to_dev(x)
+    before = big.clone()
+    out = be.forward_to_montgomery(big[:, 5:37], 3)
+    assert torch.equal(big, before)
+    want = O.ntt_forward(x, fwd[3:35], np.ascontiguousarray(params[:, 3:35]), to_montgomery=True)
+    if mode == "fast":
+        want = want % q[None, :, None]
+    np.testing.assert_array_equal(H.to_np(out), want)
