@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) per algorithmic
 # byte of the fused key switch on this workload; the rocprofv3 --pmc passes
 # and the arithmetic are recorded in profiles/keyswitch_fused_traffic.md.
-KS_TRAFFIC_BYTES_PER_ALG_BYTE = 4.103
+KS_TRAFFIC_BYTES_PER_ALG_BYTE = 3.579
 
 
 def parse_args():
@@ -305,7 +305,7 @@ def main() -> int:
         "roofline": {
             "bound": "hbm",
             "kernel": "fused hybrid key switch (fh_keyswitch_fused: k_ks_decompose, k_ks_modup_columns, "
-                      "k_ks_tiles_mac, k_inv_columns/k_inv_tiles, k_ks_moddown_add)",
+                      "k_ks_tiles_mac (with the inverse tile stages), k_inv_columns, k_ks_moddown_add)",
             "achieved": ks_achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -99,7 +99,7 @@ class HybridKeySwitcher:
                          level: int, base0: torch.Tensor | None = None,
                          base1: torch.Tensor | None = None,
                          galois_elements: list[int] | None = None) -> torch.Tensor:
-        """Whole key switch in six launches; returns [2, *batch, Q_level, N]
+        """Whole key switch in eight launches; returns [2, *batch, Q_level, N]
         holding (base0 + corr0, base1 + corr1) canonically added, or the bare
         corrections where a base is None.
 

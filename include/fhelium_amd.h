@@ -169,7 +169,7 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
                                            int64_t key_stride_row, fh_table2 rns_params,
                                            int64_t key_row_start, int64_t batch, int64_t limbs,
                                            int64_t n, void* stream);
-/* A whole hybrid key switch of a batched component in six launches
+/* A whole hybrid key switch of a batched component in eight launches
  * (decompose, ModUp fused into the forward NTT column pass, tile pass fused
  * with the key MAC over every digit, inverse NTT, ModDown fused with the
  * engine's final canonical addition).  Replaces the per-digit loop of
