@@ -33,6 +33,13 @@ HBM_PEAK_GBS = 8000.0
 # byte of the fused key switch on this workload; the rocprofv3 --pmc passes
 # and the arithmetic are recorded in profiles/keyswitch_fused_traffic.md.
 KS_TRAFFIC_BYTES_PER_ALG_BYTE = 3.579
+# Measured VALU wave-instructions (SQ_INSTS_VALU) per algorithmic byte of the
+# fused key switch on this workload (profiles/keyswitch_fused_valu_issue.md).
+KS_VALU_INSTS_PER_ALG_BYTE = 0.37768
+# gfx950 issues one wave64 VALU instruction per CU per cycle for full-rate
+# operations (the rate behind the 78.6 TFLOP/s binary64 FMA peak):
+# 256 CUs x 2.4 GHz.
+VALU_PEAK_GINSTS = 256 * 2.4
 
 
 def parse_args():
@@ -315,6 +322,16 @@ def main() -> int:
             "calls": ks_calls,
             "avg_call_ms": ks_ms / ks_calls if ks_calls else None,
             "algorithmic_bytes_per_call": (ks_bytes / ks_calls) if ks_calls else None,
+        },
+        "valu_roofline": {
+            "bound": "valu",
+            "kernel": "fused hybrid key switch (all eight launches)",
+            "achieved": (KS_VALU_INSTS_PER_ALG_BYTE * ks_bytes / (ks_ms * 1e-3) / 1e9) if ks_ms > 0 else None,
+            "peak": VALU_PEAK_GINSTS,
+            "unit": "G VALU wave-instructions/s",
+            "frac": (KS_VALU_INSTS_PER_ALG_BYTE * ks_bytes / (ks_ms * 1e-3) / 1e9 / VALU_PEAK_GINSTS)
+                    if ks_ms > 0 else None,
+            "valu_instructions_per_algorithmic_byte": KS_VALU_INSTS_PER_ALG_BYTE,
         },
         "ntt_roofline": {
             "bound": "hbm", "kernel": "fh_ntt_forward / fh_ntt_inverse (two-pass radix-16)",
