@@ -13,6 +13,11 @@ s ~ U{1..1024} with that step's direct key (all keys HBM-resident).
 One "step" is one pass over all C pairs; value = (HMults + rotations) per
 second in batched mode.  Inputs are encrypted and resident in HBM before
 timing.  Prints one JSON line.
+
+After the timed steps, a serving run (``serving`` in the JSON line) feeds
+Poisson arrivals of encrypted evaluations (a degree-7 Chebyshev polynomial
+and a 64-slot window sum) to the continuous-batching loop and reports
+throughput and p50/p99 latency.
 """
 
 from __future__ import annotations
@@ -53,6 +58,9 @@ def parse_args():
     p.add_argument("--skip-plain", action="store_true", help="time only the batched mode (profiling)")
     p.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    p.add_argument("--serve-requests", type=int, default=300,
+                   help="requests in the serving run (0 skips it)")
+    p.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     p.add_argument("--calibration-copy", action="store_true",
                    help="run one 256 MiB device copy (torch.neg) before timing, as the byte "
                         "reference for hardware-counter traffic figures")
@@ -193,6 +201,53 @@ def cpu_baseline(eng, seconds_budget: float = 30.0):
     return {"value": ops / dt, "unit": "ops/s", "cores": threads, "kind": "port",
             "sample": f"{ops // 2} x (HMult at level {level} + rotation at level {level + 1}), "
                       "logN=16 L34, oracle C restatement with OpenMP over RNS rows"}
+
+
+def serving_run(eng, args) -> dict:
+    """Serving loop: Poisson arrivals of degree-7 Chebyshev (sigmoid-like,
+    seeded) evaluations on uniform[-4, 4] inputs plus a 64-slot window sum,
+    10% of them at a reduced start level; keys resident; memory admission."""
+
+    from fhelium_amd.serving import (PolynomialWindowSum, Request, ServingLoop, logistic_chebyshev,
+                                     poisson_arrivals)
+
+    rng = np.random.default_rng(args.seed + 5)
+    prog = PolynomialWindowSum(eng, logistic_chebyshev(args.seed), radius=4.0, window=64)
+    reduced_level = 16
+    pool = {}
+    for lvl, k in ((0, 12), (reduced_level, 4)):
+        xs = rng.uniform(-4, 4, size=(k, eng.config.slots))
+        pool[lvl] = [(x, eng.encrypt(x, level=lvl)) for x in xs]
+    n = args.serve_requests
+    levels = np.where(rng.uniform(size=n) < 0.1, reduced_level, 0)
+    arrivals = poisson_arrivals(n, args.serve_rate, rng)
+
+    def requests(count, times):
+        return [Request(i, pool[int(levels[i])][i % len(pool[int(levels[i])])][1], float(times[i]))
+                for i in range(count)]
+
+    loop = ServingLoop(prog, max_batch=16, max_active=64)
+    loop.run(requests(min(n, 8), np.zeros(n)))  # warm-up: kernels and constant plaintexts
+    torch.cuda.synchronize()
+    reqs = requests(n, arrivals)
+    stats = loop.run(reqs)
+    errs = []
+    for r in reqs[:4]:
+        x = pool[r.start_level][r.rid % len(pool[r.start_level])][0]
+        errs.append(float(np.abs(eng.decrypt(r.output).real - prog.expected(x)).max()))
+    log(f"serving: {stats.completed} requests in {stats.wall_seconds:.2f}s, "
+        f"{stats.throughput:.1f} req/s, p50 {stats.percentile(50) * 1e3:.1f} ms, "
+        f"p99 {stats.percentile(99) * 1e3:.1f} ms, max active {stats.max_active}")
+    return {
+        "workload": "Poisson arrivals of degree-7 Chebyshev (sigmoid-like, seeded) on uniform[-4,4] "
+                    "plus a 64-slot window sum (6 rotate-and-add steps); 10% at start level 16",
+        "requests": stats.completed, "arrival_rate_per_s": args.serve_rate,
+        "throughput_per_s": stats.throughput, "p50_ms": stats.percentile(50) * 1e3,
+        "p99_ms": stats.percentile(99) * 1e3, "max_active": stats.max_active,
+        "max_batch": stats.max_batch_seen, "iterations": stats.iterations,
+        "key_switches_per_request": 12, "max_abs_error_sample": max(errs),
+        "error_bound": 64 * 2e-5,
+    }
 
 
 def main() -> int:
@@ -345,6 +400,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_achieved / HBM_PEAK_GBS,
         },
     }
+    if args.serve_requests > 0:
+        result["serving"] = serving_run(eng, args)
     if args.cpu_baseline:
         log("cpu baseline (oracle) ...")
         result["cpu_baseline"] = cpu_baseline(eng)

@@ -39,7 +39,7 @@ class CkksEngine:
         self.device = self.runtime.device
         self.layout = self.runtime.layout
         self.key_switcher = HybridKeySwitcher(self.runtime)
-        # fused: six-launch key switch (fh_keyswitch_fused); otherwise the
+        # fused: eight-launch key switch (fh_keyswitch_fused); otherwise the
         # reference's per-digit op sequence.  Both are bit-identical.
         self.fused_keyswitch = fused_keyswitch
         self.allow_secret_key_generation = allow_secret_key_generation
@@ -361,6 +361,40 @@ class CkksEngine:
         out = K.rescale_components(ct.data, inv, remaining, dropped_prime // 2)
         return Ciphertext(out, next_level, ct.scale / float(dropped_prime), ct.polynomial_domain,
                           ct.residue_representation, ct.modulus_basis)
+
+    def mod_switch_to_level(self, ct: Ciphertext, target_level: int) -> Ciphertext:
+        """Restrict ``ct`` to the basis at ``target_level`` by dropping its
+        leading Q rows; scale and every other state axis are kept, and the
+        result owns its storage (ckks_engine.py:1278-1320, 3209-3262)."""
+
+        if not isinstance(target_level, int):
+            raise TypeError("target_level must be an integer")
+        self.layout.check_level(target_level)
+        if target_level < ct.level:
+            raise ValueError(f"cannot mod-switch from level {ct.level} back to {target_level}")
+        drop = target_level - ct.level
+        data = ct.data[..., drop:, :].contiguous()
+        return Ciphertext(data, target_level, ct.scale, ct.polynomial_domain, ct.residue_representation,
+                          ct.modulus_basis, ct.context_id)
+
+    def reinterpret_at_scale(self, ct: Ciphertext, target_scale: float, *,
+                             max_relative_change: float | None = None) -> Ciphertext:
+        """Same residues read at ``target_scale``; with a bound, the symmetric
+        relative change max(old/new, new/old) - 1 must not exceed it
+        (ckks_engine.py:1346-1393, 3264-3300)."""
+
+        target = float(target_scale)
+        if not (math.isfinite(target) and target > 0):
+            raise InvalidScaleError(f"target scale must be positive and finite, got {target_scale}")
+        if max_relative_change is not None:
+            limit = float(max_relative_change)
+            if not (math.isfinite(limit) and limit >= 0.0):
+                raise ValueError("max_relative_change must be a non-negative finite float")
+            if max(ct.scale / target, target / ct.scale) - 1.0 > limit:
+                raise ScaleMismatchError(f"reinterpret_at_scale: {ct.scale} -> {target} exceeds {limit}")
+        out = ct.clone()
+        out.scale = target
+        return out
 
     def _assert_rotation_ciphertext(self, ct: Ciphertext) -> None:
         ct.assert_state(polynomial_domain="coefficient", modulus_basis="Q", residue_representation="standard",
