@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import enum
 import functools
+import hashlib
 import json
 import math
 import pathlib
@@ -146,6 +147,20 @@ class CkksConfig:
     @property
     def default_scale(self) -> float:
         return float(1 << self.scale_bits)
+
+    @property
+    def context_id(self) -> str:
+        """SHA-256 fingerprint of the mathematical context, computed from the
+        same payload text as the reference (fhelium/core/context.py:72-84),
+        so value files name the same context on both backends."""
+
+        payload = (
+            "representation=direct_per_value_scale_v1;"
+            f"logN={self.logN};default_scale={self.default_scale!r};"
+            f"Q={','.join(str(q) for q in self.q_moduli)};"
+            f"P={','.join(str(p) for p in self.p_moduli)};galois_generator=3"
+        )
+        return hashlib.sha256(payload.encode("utf-8")).hexdigest()
 
     @property
     def torch_dtype(self) -> torch.dtype:

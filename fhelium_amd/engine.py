@@ -91,6 +91,36 @@ class CkksEngine:
         self.secret_key = SecretKey(data=data, coefficients=s)
         return self.secret_key
 
+    def set_secret_key(self, secret_key: SecretKey) -> SecretKey:
+        """Install a secret key (e.g. one read from a value file).  Its signed
+        coefficients, needed for rotation keys, are recovered from the first
+        NTT row when absent."""
+
+        cfg = self.config
+        if tuple(secret_key.data.shape) != (cfg.total_num_primes, cfg.N):
+            raise ValueError("secret key must be [QP, N]")
+        data = secret_key.data.to(self.device)
+        coefficients = secret_key.coefficients
+        if coefficients is None:
+            std = self.runtime.ntt.inverse_to_standard(data[:1].contiguous(), 0)[0]
+            q0 = cfg.moduli[0]
+            coefficients = torch.where(std > q0 // 2, std - q0, std)
+        self.secret_key = SecretKey(data=data, coefficients=coefficients.to(self.device))
+        return self.secret_key
+
+    def set_relinearization_key(self, key: RelinearizationKey) -> RelinearizationKey:
+        self.relinearization_key = RelinearizationKey(key.data.to(self.device))
+        return self.relinearization_key
+
+    def set_rotation_key(self, key: RotationKey) -> RotationKey:
+        step = RotationKey.canonical_step(key.rotation_step, self.config.N)
+        g = rns.rotation_galois_element(self.config.N, step, self.galois_generator)
+        if key.galois_element != g:
+            raise ValueError(f"rotation key for step {step} carries Galois element {key.galois_element} != {g}")
+        stored = RotationKey(key.data.to(self.device), rotation_step=step, galois_element=g)
+        self.rotation_keys[step] = stored
+        return stored
+
     def _require_secret(self) -> SecretKey:
         if self.secret_key is None:
             if not self.allow_secret_key_generation:

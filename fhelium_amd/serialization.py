@@ -1,0 +1,305 @@
+"""Value files: one exact value per safetensors file.
+
+The on-disk layout is the reference's value-file format
+(fhelium/serialization/safetensors.py:23-130, value.py:29-420), so keys and
+ciphertexts written by either backend load on the other:
+
+* safetensors metadata ``fhelium.format = "fhelium-value"``,
+  ``fhelium.schema_version = "1"`` and ``fhelium.manifest``, a compact,
+  key-sorted JSON object with ``file_schema_version`` (1),
+  ``value_schema_version`` (2), ``value_type``, ``context_id``, ``nbytes``,
+  ``tensor_metadata`` and ``value_metadata``;
+* one tensor payload per logical tensor (``encoding = "direct"``), stored
+  contiguous on the host;
+* ``value_metadata`` per type: ``Ciphertext`` {level, scale, prime_ids,
+  polynomial_domain, modulus_basis, residue_representation};
+  ``Plaintext`` (RNS form) adds representation = "rns", has_message = false,
+  has_data = true; keys {prime_ids, polynomial_domain, modulus_basis,
+  residue_representation} plus ``rotation_step`` for rotation keys.
+
+``context_id`` is the SHA-256 context fingerprint (``CkksConfig.context_id``).
+Values here do not carry their context, so ``save_value`` takes it as
+``context`` (a ``CkksConfig`` or an engine).  Files are written atomically
+(temporary file, then rename) with mode 0600; secret keys need
+``allow_secret=True``.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+import tempfile
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Any
+
+import torch
+from safetensors import safe_open
+from safetensors.torch import save_file
+
+from fhelium_amd import rns
+from fhelium_amd.core import (Ciphertext, KeySwitchKey, Plaintext, RelinearizationKey, RotationKey,
+                              SecretKey)
+
+FILE_FORMAT = "fhelium-value"
+FILE_SCHEMA_VERSION = 1
+VALUE_SCHEMA_VERSION = 2
+_FORMAT_KEY = "fhelium.format"
+_SCHEMA_KEY = "fhelium.schema_version"
+_MANIFEST_KEY = "fhelium.manifest"
+_KEY_TYPES = ("SecretKey", "KeySwitchKey", "RotationKey", "RelinearizationKey")
+_VALUE_TYPES = ("Plaintext", "Ciphertext") + _KEY_TYPES
+_DTYPES = {"torch.int64": "I64", "torch.int32": "I32"}
+
+
+@dataclass(frozen=True)
+class ValueFileMetadata:
+    file_schema_version: int
+    value_schema_version: int
+    value_type: str
+    context_id: str | None
+    nbytes: int
+    tensor_metadata: dict[str, dict[str, Any]]
+    value_metadata: dict[str, Any]
+
+
+def _config_of(context):
+    return getattr(context, "config", context)
+
+
+def _q_prime_ids(cfg, level: int, basis: str) -> list[int]:
+    ids = list(range(level, cfg.num_q_primes))
+    if basis == "QP":
+        ids += list(range(cfg.num_q_primes, cfg.total_num_primes))
+    return ids
+
+
+def _describe(value, cfg) -> tuple[str, dict[str, Any], torch.Tensor]:
+    all_rows = list(range(cfg.total_num_primes))
+    if isinstance(value, Ciphertext):
+        return "Ciphertext", {
+            "level": value.level, "scale": float(value.scale),
+            "prime_ids": _q_prime_ids(cfg, value.level, value.modulus_basis),
+            "polynomial_domain": value.polynomial_domain, "modulus_basis": value.modulus_basis,
+            "residue_representation": value.residue_representation,
+        }, value.data
+    if isinstance(value, Plaintext):
+        return "Plaintext", {
+            "level": value.level, "scale": float(value.scale), "representation": "rns",
+            "polynomial_domain": value.polynomial_domain, "modulus_basis": value.modulus_basis,
+            "residue_representation": value.residue_representation,
+            "prime_ids": _q_prime_ids(cfg, value.level, value.modulus_basis),
+            "has_message": False, "has_data": True,
+        }, value.data
+    key_meta = {"prime_ids": all_rows, "polynomial_domain": "ntt", "modulus_basis": "QP",
+                "residue_representation": "montgomery"}
+    if isinstance(value, SecretKey):
+        return "SecretKey", key_meta, value.data
+    if isinstance(value, RotationKey):
+        return "RotationKey", dict(key_meta, rotation_step=int(value.rotation_step)), value.data
+    if isinstance(value, RelinearizationKey):
+        return "RelinearizationKey", key_meta, value.data
+    if isinstance(value, KeySwitchKey):
+        return "KeySwitchKey", key_meta, value.data
+    raise TypeError(f"value files hold {list(_VALUE_TYPES)}; got {type(value).__name__}")
+
+
+def save_value(value, path: str | os.PathLike[str], *, context, allow_secret: bool = False,
+               overwrite: bool = False) -> ValueFileMetadata:
+    """Write ``value`` to ``path`` (fhelium/serialization/safetensors.py:47-130)."""
+
+    cfg = _config_of(context)
+    value_type, value_meta, data = _describe(value, cfg)
+    if value_type == "SecretKey" and not allow_secret:
+        raise PermissionError("SecretKey persistence is disabled by default; pass allow_secret=True "
+                              "only for a path with an at-rest security policy")
+    dest = Path(path).expanduser()
+    if not dest.parent.exists():
+        raise FileNotFoundError(f"Value file parent directory does not exist: {dest.parent}")
+    if not dest.parent.is_dir():
+        raise NotADirectoryError(dest.parent)
+    if dest.exists():
+        if dest.is_dir():
+            raise IsADirectoryError(dest)
+        if not overwrite:
+            raise FileExistsError(dest)
+    host = data.detach().to("cpu").contiguous()
+    if str(host.dtype) not in _DTYPES:
+        raise TypeError(f"value payloads are int64 or int32 residues, got {host.dtype}")
+    manifest = {
+        "file_schema_version": FILE_SCHEMA_VERSION,
+        "value_schema_version": VALUE_SCHEMA_VERSION,
+        "value_type": value_type,
+        "context_id": cfg.context_id,
+        "nbytes": host.numel() * host.element_size(),
+        "tensor_metadata": {"data": {"shape": list(host.shape), "dtype": str(host.dtype),
+                                     "encoding": "direct", "payloads": ["data"]}},
+        "value_metadata": value_meta,
+    }
+    meta = _parse_manifest(manifest)
+    fd, tmp_name = tempfile.mkstemp(prefix=f".{dest.name}.", suffix=".tmp", dir=dest.parent)
+    os.close(fd)
+    tmp = Path(tmp_name)
+    try:
+        save_file({"data": host}, str(tmp), metadata={
+            _FORMAT_KEY: FILE_FORMAT, _SCHEMA_KEY: str(FILE_SCHEMA_VERSION),
+            _MANIFEST_KEY: json.dumps(manifest, sort_keys=True, separators=(",", ":"))})
+        os.chmod(tmp, 0o600)
+        if dest.exists() and not overwrite:
+            raise FileExistsError(dest)
+        os.replace(tmp, dest)
+    except BaseException:
+        tmp.unlink(missing_ok=True)
+        raise
+    return meta
+
+
+def _parse_manifest(manifest: Any) -> ValueFileMetadata:
+    fields = {"file_schema_version", "value_schema_version", "value_type", "context_id", "nbytes",
+              "tensor_metadata", "value_metadata"}
+    if not isinstance(manifest, dict):
+        raise ValueError("Value file manifest must be a JSON object")
+    if set(manifest) != fields:
+        raise ValueError(f"Value file manifest fields differ: {sorted(set(manifest) ^ fields)}")
+    for name, want in (("file_schema_version", FILE_SCHEMA_VERSION),
+                       ("value_schema_version", VALUE_SCHEMA_VERSION)):
+        if type(manifest[name]) is not int or manifest[name] != want:
+            raise ValueError(f"Unsupported value file {name.replace('_', ' ')}: {manifest[name]!r}")
+    vt = manifest["value_type"]
+    if vt not in _VALUE_TYPES:
+        raise ValueError(f"Unsupported value file type: {vt!r}")
+    ctx = manifest["context_id"]
+    if not isinstance(ctx, str) or not ctx:
+        raise ValueError("Value file context_id must be a non-empty string")
+    if type(manifest["nbytes"]) is not int or manifest["nbytes"] < 0:
+        raise ValueError("Value file nbytes must be a non-negative integer")
+    tmeta, vmeta = manifest["tensor_metadata"], manifest["value_metadata"]
+    if not isinstance(tmeta, dict) or not isinstance(vmeta, dict):
+        raise ValueError("Value file metadata fields must be JSON objects")
+    if set(tmeta) != {"data"}:
+        raise ValueError(f"{vt} value files hold exactly one tensor named 'data'")
+    item = tmeta["data"]
+    if (not isinstance(item, dict) or set(item) != {"shape", "dtype", "encoding", "payloads"}
+            or not isinstance(item["shape"], list)
+            or any(type(d) is not int or d < 0 for d in item["shape"])
+            or item["dtype"] not in _DTYPES or item["encoding"] != "direct" or item["payloads"] != ["data"]):
+        raise ValueError("Value tensor metadata for 'data' is invalid")
+    _check_value_metadata(vt, vmeta, item["shape"])
+    return ValueFileMetadata(FILE_SCHEMA_VERSION, VALUE_SCHEMA_VERSION, vt, ctx, manifest["nbytes"], tmeta,
+                             vmeta)
+
+
+def _check_value_metadata(vt: str, m: dict[str, Any], shape: list[int]) -> None:
+    need = {"prime_ids", "polynomial_domain", "modulus_basis", "residue_representation"}
+    if vt == "Ciphertext":
+        need |= {"level", "scale"}
+    elif vt == "Plaintext":
+        need |= {"level", "scale", "representation", "has_message", "has_data"}
+    elif vt == "RotationKey":
+        need |= {"rotation_step"}
+    if set(m) != need:
+        raise ValueError(f"{vt} metadata fields differ: {sorted(set(m) ^ need)}")
+    if m["polynomial_domain"] not in ("coefficient", "ntt") or m["modulus_basis"] not in ("Q", "QP") \
+            or m["residue_representation"] not in ("standard", "montgomery"):
+        raise ValueError(f"{vt} metadata has an unsupported state")
+    if m["polynomial_domain"] == "ntt" and m["residue_representation"] != "montgomery":
+        raise ValueError(f"{vt}: NTT-domain values hold Montgomery residues")
+    ids = m["prime_ids"]
+    if not isinstance(ids, list) or any(type(i) is not int or i < 0 for i in ids) or len(set(ids)) != len(ids):
+        raise ValueError(f"{vt} prime_ids must be distinct non-negative integers")
+    if not shape or len(ids) != (shape[-2] if len(shape) >= 2 else -1):
+        raise ValueError(f"{vt} prime_ids do not match the limb axis of the payload")
+    if vt in ("Ciphertext", "Plaintext"):
+        if type(m["level"]) is not int or m["level"] < 0:
+            raise ValueError(f"{vt} level must be a non-negative integer")
+        if not isinstance(m["scale"], (int, float)) or isinstance(m["scale"], bool) \
+                or not math.isfinite(m["scale"]) or m["scale"] <= 0:
+            raise ValueError(f"{vt} scale must be positive and finite")
+    if vt == "Ciphertext" and m["polynomial_domain"] == "coefficient" and m["residue_representation"] != "standard":
+        raise ValueError("Coefficient-domain Ciphertext requires standard residues")
+    if vt == "Plaintext" and (m["representation"] != "rns" or m["has_message"] is not False
+                              or m["has_data"] is not True):
+        raise ValueError("only RNS-form Plaintext files are supported")
+    if vt == "RotationKey" and type(m["rotation_step"]) is not int:
+        raise ValueError("RotationKey rotation_step must be an integer")
+
+
+def _read_metadata(handle) -> ValueFileMetadata:
+    raw = handle.metadata()
+    if raw is None:
+        raise ValueError("Value file is missing safetensors metadata")
+    if raw.get(_FORMAT_KEY) != FILE_FORMAT:
+        raise ValueError(f"Unsupported value file format: {raw.get(_FORMAT_KEY)!r}")
+    if raw.get(_SCHEMA_KEY) != str(FILE_SCHEMA_VERSION):
+        raise ValueError(f"Unsupported value file schema version: {raw.get(_SCHEMA_KEY)!r}")
+    if _MANIFEST_KEY not in raw:
+        raise ValueError("Value file is missing its embedded manifest")
+    try:
+        manifest = json.loads(raw[_MANIFEST_KEY])
+    except json.JSONDecodeError as error:
+        raise ValueError("Value file manifest is not valid JSON") from error
+    meta = _parse_manifest(manifest)
+    if set(handle.keys()) != {"data"}:
+        raise ValueError("Value file payload set does not match its manifest")
+    sl = handle.get_slice("data")
+    item = meta.tensor_metadata["data"]
+    if tuple(sl.get_shape()) != tuple(item["shape"]) or sl.get_dtype() != _DTYPES[item["dtype"]]:
+        raise ValueError("Value file payload header does not match logical tensor metadata for 'data'")
+    return meta
+
+
+def inspect_value(path: str | os.PathLike[str]) -> ValueFileMetadata:
+    """Validated metadata without reading the payload (safetensors.py:132-147)."""
+
+    src = Path(path).expanduser()
+    if not src.is_file():
+        raise FileNotFoundError(f"No value file at {src}")
+    with safe_open(str(src), framework="pt", device="cpu") as handle:
+        return _read_metadata(handle)
+
+
+def load_value(path: str | os.PathLike[str], *, context, device: torch.device | str = "cpu",
+               expected_type: type | None = None, check_context: bool = True):
+    """Read one value (safetensors.py:150-219).  ``context`` (a ``CkksConfig``
+    or an engine) supplies the ring for Galois elements and, with
+    ``check_context``, must have the file's context fingerprint."""
+
+    cfg = _config_of(context)
+    src = Path(path).expanduser()
+    if not src.is_file():
+        raise FileNotFoundError(f"No value file at {src}")
+    with safe_open(str(src), framework="pt", device="cpu") as handle:
+        meta = _read_metadata(handle)
+        if check_context and meta.context_id != cfg.context_id:
+            raise ValueError(f"Value file context mismatch: expected {cfg.context_id!r}, got "
+                             f"{meta.context_id!r}")
+        if expected_type is not None and meta.value_type != expected_type.__name__:
+            raise TypeError(f"Value file has type {meta.value_type}, expected {expected_type.__name__}")
+        data = handle.get_tensor("data")
+    if data.numel() * data.element_size() != meta.nbytes:
+        raise ValueError("Value file byte count mismatch")
+    data = data.to(device)
+    m = meta.value_metadata
+    vt = meta.value_type
+    if vt == "Ciphertext":
+        if m["prime_ids"] != _q_prime_ids(cfg, m["level"], m["modulus_basis"]):
+            raise ValueError("Ciphertext prime_ids are not this context's rows for its level")
+        return Ciphertext(data, m["level"], float(m["scale"]), m["polynomial_domain"],
+                          m["residue_representation"], m["modulus_basis"], meta.context_id)
+    if vt == "Plaintext":
+        if m["prime_ids"] != _q_prime_ids(cfg, m["level"], m["modulus_basis"]):
+            raise ValueError("Plaintext prime_ids are not this context's rows for its level")
+        return Plaintext(data, m["level"], float(m["scale"]), m["polynomial_domain"],
+                         m["residue_representation"], m["modulus_basis"])
+    if m["prime_ids"] != list(range(cfg.total_num_primes)) or m["modulus_basis"] != "QP" \
+            or m["polynomial_domain"] != "ntt" or m["residue_representation"] != "montgomery":
+        raise ValueError(f"{vt} must hold every QP row in the NTT domain with Montgomery residues")
+    if vt == "SecretKey":
+        return SecretKey(data=data, coefficients=None)
+    if vt == "RotationKey":
+        step = RotationKey.canonical_step(m["rotation_step"], cfg.N)
+        return RotationKey(data, rotation_step=step, galois_element=rns.rotation_galois_element(cfg.N, step, 3))
+    if vt == "RelinearizationKey":
+        return RelinearizationKey(data)
+    return KeySwitchKey(data)
