@@ -5,7 +5,8 @@
 // csrc/ops/rns/cpu/rns_arithmetic_cpu.cpp:482-700 and
 // csrc/ops/ckks/cpu/ckks_cpu.cpp:474-708), reorganised into eight launches
 // instead of ~100 (ModUp and the tile pass launch once for the binary64 rows
-// and once for the integer rows):
+// and once for the integer rows; the integer rows' two launches run on a
+// side stream, concurrently with the binary64 rows' launches):
 //
 //   1. k_ks_decompose     Garner digits of every active hybrid digit (one
 //                         launch for all digits); optionally reads its input
@@ -38,6 +39,7 @@
 // unique canonical values, and ModDown (an exact floor division) and the
 // final canonical addition are exact functions of those.  The outputs are
 // therefore bit-identical to the reference CPU backend's in both modes.
+#include <mutex>
 #include <type_traits>
 
 #include "fh_common.h"
@@ -739,6 +741,45 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   }
 }
 
+// The integer-row (60-bit prime) ModUp and MAC launches are VALU-bound and
+// the binary64-row MAC is HBM-bound, so the two paths run concurrently: the
+// integer rows on a side stream forked from the caller's stream by an event
+// and joined back before the inverse column pass.  One side stream and one
+// fork/join event pair per device, created on first use; the mutex keeps
+// the pair to one fused call at a time.
+struct SideStream {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr;
+  hipEvent_t join = nullptr;
+};
+constexpr int kMaxDevices = 64;
+SideStream g_side[kMaxDevices];
+std::mutex g_side_mutex;
+
+int side_stream(SideStream** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("fused key switch: no usable current device (%d)", dev);
+    return kLaunchFailed;
+  }
+  SideStream& sd = g_side[dev];
+  if (sd.stream == nullptr) {
+    hipStream_t st = nullptr;
+    hipEvent_t f = nullptr, j = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+      set_error("fused key switch: side stream creation failed: %s", hipGetErrorString(hipGetLastError()));
+      return kLaunchFailed;
+    }
+    sd.stream = st;
+    sd.fork = f;
+    sd.join = j;
+  }
+  *out = &sd;
+  return kOk;
+}
+
 }  // namespace
 }  // namespace fh
 
@@ -812,20 +853,23 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
 
   const int T = (1 << s1) / 16;
   const int C = kBlock / T;
-  for (int pass = 0; pass < 2; ++pass) {
+  const int64_t groups = n / (kTile * kMacTilesPerBlock);
+  FH_CHECK_ARG(groups * lqp * batch <= 0x7fffffff, "key switch grid too large");
+  // pass 0: binary64 rows [0, small_rows); pass 1: integer rows.
+  auto launch_modup = [&](int pass, hipStream_t st) -> int {
     const int64_t j_begin = pass == 0 ? 0 : small_rows;
     const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
-    if (j_count == 0) continue;
+    if (j_count == 0) return kOk;
     const dim3 gcol(kTile / C, static_cast<uint32_t>(j_count), static_cast<uint32_t>(digit_count * batch));
-#define FH_MODUP_CASE(S)                                                                          \
-  case S:                                                                                         \
-    if (pass == 1) {                                                                              \
-      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, false>), gcol, dim3(kBlock), 0, s, a, j_begin); \
-    } else if (mode == 1) {                                                                       \
-      hipLaunchKernelGGL((k_ks_modup_columns<S, 1, true>), gcol, dim3(kBlock), 0, s, a, j_begin);  \
-    } else {                                                                                      \
-      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, true>), gcol, dim3(kBlock), 0, s, a, j_begin);  \
-    }                                                                                             \
+#define FH_MODUP_CASE(S)                                                                           \
+  case S:                                                                                          \
+    if (pass == 1) {                                                                               \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, false>), gcol, dim3(kBlock), 0, st, a, j_begin); \
+    } else if (mode == 1) {                                                                        \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 1, true>), gcol, dim3(kBlock), 0, st, a, j_begin);  \
+    } else {                                                                                       \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, true>), gcol, dim3(kBlock), 0, st, a, j_begin);  \
+    }                                                                                              \
     break;
     switch (s1) {
       FH_MODUP_CASE(4)
@@ -836,26 +880,57 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
         FH_MODUP_CASE(8)
     }
 #undef FH_MODUP_CASE
-  }
-  FH_CHECK_LAUNCH("key switch modup + column pass");
-
-  const int64_t groups = n / (kTile * kMacTilesPerBlock);
-  for (int pass = 0; pass < 2; ++pass) {
+    FH_CHECK_LAUNCH("key switch modup + column pass");
+    return kOk;
+  };
+  auto launch_mac = [&](int pass, hipStream_t st) -> int {
     const int64_t j_begin = pass == 0 ? 0 : small_rows;
     const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
-    if (j_count == 0) continue;
-    const int64_t total = groups * j_count * batch;
-    FH_CHECK_ARG(total <= 0x7fffffff, "key switch grid too large");
-    const dim3 grid(static_cast<uint32_t>(total));
+    if (j_count == 0) return kOk;
+    const dim3 grid(static_cast<uint32_t>(groups * j_count * batch));
     if (pass == 1) {
-      hipLaunchKernelGGL((k_ks_tiles_mac<0, false>), grid, dim3(kBlock), 0, s, a, groups, j_begin, j_count);
+      hipLaunchKernelGGL((k_ks_tiles_mac<0, false>), grid, dim3(kBlock), 0, st, a, groups, j_begin, j_count);
     } else if (mode == 1) {
-      hipLaunchKernelGGL((k_ks_tiles_mac<1, true>), grid, dim3(kBlock), 0, s, a, groups, j_begin, j_count);
+      hipLaunchKernelGGL((k_ks_tiles_mac<1, true>), grid, dim3(kBlock), 0, st, a, groups, j_begin, j_count);
     } else {
-      hipLaunchKernelGGL((k_ks_tiles_mac<0, true>), grid, dim3(kBlock), 0, s, a, groups, j_begin, j_count);
+      hipLaunchKernelGGL((k_ks_tiles_mac<0, true>), grid, dim3(kBlock), 0, st, a, groups, j_begin, j_count);
+    }
+    FH_CHECK_LAUNCH("key switch tile pass + MAC");
+    return kOk;
+  };
+  {
+    std::lock_guard<std::mutex> lock(g_side_mutex);
+    SideStream* side = nullptr;
+    int rc = side_stream(&side);
+    if (rc != kOk) return rc;
+    const bool split = small_rows > 0 && small_rows < lqp;
+    if (split) {
+      if (hipEventRecord(side->fork, s) != hipSuccess || hipStreamWaitEvent(side->stream, side->fork, 0) != hipSuccess) {
+        set_error("fused key switch: fork failed: %s", hipGetErrorString(hipGetLastError()));
+        return kLaunchFailed;
+      }
+      rc = launch_modup(1, side->stream);
+      if (rc == kOk) rc = launch_mac(1, side->stream);
+      if (rc != kOk) return rc;
+      if (hipEventRecord(side->join, side->stream) != hipSuccess) {
+        set_error("fused key switch: join record failed: %s", hipGetErrorString(hipGetLastError()));
+        return kLaunchFailed;
+      }
+      rc = launch_modup(0, s);
+      if (rc == kOk) rc = launch_mac(0, s);
+      if (rc != kOk) return rc;
+      if (hipStreamWaitEvent(s, side->join, 0) != hipSuccess) {
+        set_error("fused key switch: join failed: %s", hipGetErrorString(hipGetLastError()));
+        return kLaunchFailed;
+      }
+    } else {
+      for (int pass = 0; pass < 2 && rc == kOk; ++pass) {
+        rc = launch_modup(pass, s);
+        if (rc == kOk) rc = launch_mac(pass, s);
+      }
+      if (rc != kOk) return rc;
     }
   }
-  FH_CHECK_LAUNCH("key switch tile pass + MAC");
 
   // Both accumulators as one [2B, Lqp, N] inverse transform.
   // The tile stages of this inverse ran inside the MAC kernel.
