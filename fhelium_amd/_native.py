@@ -81,6 +81,9 @@ SIGNATURES: dict[str, list] = {
     "fh_keyswitch_fused": [fh_view3, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, _i64, _ptr, _ptr, fh_table2, _ptr,
                            _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
                            _ptr, fh_view3, fh_view3, fh_view3, fh_view3, _i32, _ptr],
+    "fh_ckks_hmult_tensor_coefficient": [_ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
+                                         _i64, _i64, fh_table2, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
+                                         _ptr],
 }
 
 _lock = threading.Lock()

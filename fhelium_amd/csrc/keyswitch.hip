@@ -617,6 +617,234 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSmall ?
 }
 
 // ---------------------------------------------------------------------------
+// HMult front: the tensor product of two coefficient-domain ciphertexts,
+// returned in the coefficient domain (what relinearization consumes).
+// The forward column passes of a0, a1, b0, b1 (to Montgomery form) run as
+// ordinary launches; this kernel then runs, per tile, the four forward tile
+// passes, the products d0 = a0 b0, d1 = a0 b1 + a1 b0, d2 = a1 b1 and the
+// first eight inverse stages of d0, d1, d2, so the NTT-domain operands and
+// products never go to HBM.  Binary64 rows multiply the Montgomery values as
+// plain residues (d = a b R^2) and integer rows with REDC (d = a b R); the
+// inverse column pass with kOutStdMac removes exactly those factors.
+// ---------------------------------------------------------------------------
+struct HmArgs {
+  const int64_t* ws;  // [4, B, L, N] column-pass outputs of a0, a1, b0, b1
+  int64_t* out;       // [3, B, L, N] working-format rows for the inverse column pass
+  int64_t batch, limbs, row_start;
+  int log_n;
+  ParamView params;   // global QP rows
+  const NttRow* rows; // global QP NTT records
+  const int64_t* fwd_w;
+  const int64_t* fwd_wp;
+  const int64_t* inv_w;
+  const int64_t* inv_wp;
+};
+
+template <class P>
+__device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t g, int64_t j, int64_t b,
+                                            int64_t* lds) {
+  using V = typename P::V;
+  constexpr bool kF64 = std::is_same<V, double>::value;
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t grow = a.row_start + j;
+  const NttRow& row = a.rows[grow];
+  const int m = threadIdx.x / kMacThreads;
+  const int t = threadIdx.x % kMacThreads;
+  const int r = static_cast<int>(g) * kMacTiles + m;
+  const int s0 = a.log_n - kTileLog;
+  int64_t* sh = lds + m * kTilePitch;
+  int64_t* tww = lds + kMacExchange + m * kTile;
+  int64_t* twp = tww + kMacTiles * kTile;  // integer rows only
+  {
+    const int64_t* w_row = a.fwd_w + grow * n;
+    const int64_t* wp_row = a.fwd_wp + grow * n;
+    for (int idx = t; idx < kTile - 1; idx += kMacThreads) {
+      const int st = 31 - __builtin_clz(idx + 1);
+      const int64_t gi = (int64_t{1} << (s0 + st)) + (static_cast<int64_t>(r) << st) + (idx + 1 - (1 << st));
+      tww[idx] = w_row[gi];
+      if constexpr (!kF64) twp[idx] = wp_row[gi];
+    }
+  }
+  __syncthreads();
+  auto tw_at = [&](int st, int e) {
+    const int slot = mac_tw_slot(st, e);
+    return p.tw_make(tww[slot], kF64 ? 0 : twp[slot]);
+  };
+  const int64_t tile_off = static_cast<int64_t>(r) * kTile;
+  const int64_t comp_stride = a.batch * a.limbs * n;
+  const int64_t* in = a.ws + (b * a.limbs + j) * n + tile_off;
+  // Forward tile stages of component c; returns the values in the e3
+  // layout (thread t holds elements 8t .. 8t+7).
+  auto forward_tile = [&](int c, V (&v)[8]) {
+    const int64_t* tile = in + c * comp_stride;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(tile[mac_e1(t, k)]);
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+      const int half = 4 >> st;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half], tw_at(st, mac_e1(t, k)));
+      }
+    }
+    __syncthreads();  // earlier readers of the exchange buffer are done
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
+#pragma unroll
+    for (int st = 3; st < 6; ++st) {
+      const int half = 1 << (5 - st);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half], tw_at(st, mac_e2(t, k)));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
+#pragma unroll
+    for (int st = 6; st < 8; ++st) {
+      const int half = 1 << (7 - st);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half], tw_at(st, mac_e3(t, k)));
+      }
+    }
+  };
+  V d0[8], d1[8], d2[8];
+  {
+    V x0[8], x1[8], y[8];
+    forward_tile(0, x0);  // a0
+    forward_tile(1, x1);  // a1
+    forward_tile(2, y);   // b0
+    if constexpr (kF64) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        x0[k] = p.reduce(x0[k]);
+        x1[k] = p.reduce(x1[k]);
+        const double yb = p.reduce(y[k]);
+        d0[k] = f64_mulmod_signed(x0[k], yb, row.qd, row.qinv);
+        d1[k] = f64_mulmod_signed(x1[k], yb, row.qd, row.qinv);
+      }
+    } else {
+      const Mont mont = load_mont(a.params, grow);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        d0[k] = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(x0[k]), static_cast<int64_t>(y[k]), mont));
+        d1[k] = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(x1[k]), static_cast<int64_t>(y[k]), mont));
+      }
+    }
+    forward_tile(3, y);   // b1
+    if constexpr (kF64) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const double yb = p.reduce(y[k]);
+        d1[k] += f64_mulmod_signed(x0[k], yb, row.qd, row.qinv);
+        d2[k] = f64_mulmod_signed(x1[k], yb, row.qd, row.qinv);
+      }
+    } else {
+      const Mont mont = load_mont(a.params, grow);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        d1[k] = static_cast<uint64_t>(add_lazy(
+            static_cast<int64_t>(d1[k]), mont_mul(static_cast<int64_t>(x0[k]), static_cast<int64_t>(y[k]), mont),
+            mont.twice_q));
+        d2[k] = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(x1[k]), static_cast<int64_t>(y[k]), mont));
+      }
+    }
+  }
+  // Inverse tile stages (the first eight of the inverse NTT) of d0, d1, d2.
+  __syncthreads();  // every thread is done with the forward twiddles
+  {
+    const int64_t* w_row = a.inv_w + grow * n;
+    const int64_t* wp_row = a.inv_wp + grow * n;
+    for (int idx = t; idx < kTile - 1; idx += kMacThreads) {
+      const int st = 7 - (31 - __builtin_clz(kTile - 1 - idx));
+      const int gg = idx - (kTile - (kTile >> st));
+      const int64_t gi = (n >> (st + 1)) + (static_cast<int64_t>(r) << (7 - st)) + gg;
+      tww[idx] = w_row[gi];
+      if constexpr (!kF64) twp[idx] = wp_row[gi];
+    }
+  }
+  __syncthreads();
+  auto inv_slot = [](int st, int e) { return (kTile - (kTile >> st)) + (e >> (st + 1)); };
+  auto inverse_tile = [&](V (&v)[8], int64_t* out) {
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+      const int half = 1 << st;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        const int slot = inv_slot(st, mac_e3(t, k));
+        p.inv(v[k], v[k + half], p.tw_make(tww[slot], kF64 ? 0 : twp[slot]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e3(t, k))] = p.mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
+#pragma unroll
+    for (int st = 3; st < 5; ++st) {
+      const int half = 1 << (st - 2);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        const int slot = inv_slot(st, mac_e2(t, k));
+        p.inv(v[k], v[k + half], p.tw_make(tww[slot], kF64 ? 0 : twp[slot]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
+#pragma unroll
+    for (int st = 5; st < 8; ++st) {
+      const int half = 1 << (st - 5);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        const int slot = inv_slot(st, mac_e1(t, k));
+        p.inv(v[k], v[k + half], p.tw_make(tww[slot], kF64 ? 0 : twp[slot]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[mac_e1(t, k)] = p.mid(v[k]);
+    __syncthreads();  // the exchange buffer is reused by the next product
+  };
+  int64_t* out = a.out + (b * a.limbs + j) * n + tile_off;
+  inverse_tile(d0, out);
+  inverse_tile(d1, out + comp_stride);
+  inverse_tile(d2, out + 2 * comp_stride);
+}
+
+template <bool kSmall>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_hmult_tiles(
+    HmArgs a, int64_t groups, int64_t j_begin, int64_t j_count) {
+  __shared__ int64_t lds[kMacExchange + kMacTiles * kTile * (kSmall ? 1 : 2)];
+  const uint32_t batch = static_cast<uint32_t>(a.batch);
+  const uint32_t jc = static_cast<uint32_t>(j_count);
+  const uint32_t w = xcd_remap(blockIdx.x, static_cast<uint32_t>(groups) * jc * batch);
+  const int64_t b = w % batch;
+  const int64_t j = j_begin + (w / batch) % jc;
+  const int64_t g = w / (batch * jc);
+  const NttRow& row = a.rows[a.row_start + j];
+  if constexpr (kSmall) {
+    tiles_hmult(F64Fast(row), a, g, j, b, lds);
+  } else {
+    tiles_hmult(U64Path<false>(row), a, g, j, b, lds);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 5. ModDown + final canonical addition.
 //    Grid (N/256, ceil(Lq / 4), 2 * B); each thread owns one coefficient of
 //    four Q rows of one accumulator.  After the inverse NTT the accumulator
@@ -963,4 +1191,65 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   hipLaunchKernelGGL(k_ks_moddown_add, dim3(static_cast<uint32_t>(md_blocks)), dim3(kBlock), 0, s, md);
   FH_CHECK_LAUNCH("key switch moddown + add");
   return kOk;
+}
+
+// Tensor product of two coefficient-domain ciphertexts, returned in the
+// coefficient domain.  See include/fhelium_amd.h.
+FH_API int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, int64_t a_comp_stride,
+                                            int64_t a_batch_stride, int64_t a_limb_stride,
+                                            int64_t b_comp_stride, int64_t b_batch_stride,
+                                            int64_t b_limb_stride, int64_t batch, int64_t limbs,
+                                            int64_t log_n, int64_t row_start, int64_t small_rows,
+                                            fh_table2 rns_params, const void* ntt_rows,
+                                            const int64_t* forward_twiddles,
+                                            const int64_t* forward_twiddles_shoup,
+                                            const int64_t* inverse_twiddles,
+                                            const int64_t* inverse_twiddles_shoup, int64_t* workspace,
+                                            int64_t* out, void* stream) {
+  FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "fused tensor product supports logN 12..16");
+  FH_CHECK_ARG(batch >= 1 && limbs >= 1 && row_start >= 0, "bad batch, limb count or row start");
+  FH_CHECK_ARG(small_rows >= 0 && small_rows <= limbs, "bad binary64 row count");
+  FH_CHECK_ARG(a && b && rns_params.data && ntt_rows && forward_twiddles && forward_twiddles_shoup &&
+                   inverse_twiddles && inverse_twiddles_shoup && workspace && out,
+               "fused tensor product: null operand");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t n = int64_t{1} << log_n;
+  const int64_t comp = batch * limbs * n;
+  const NttRow* rows = static_cast<const NttRow*>(ntt_rows);
+  const int64_t* srcs[4] = {a, a + a_comp_stride, b, b + b_comp_stride};
+  const int64_t sbs[4] = {a_batch_stride, a_batch_stride, b_batch_stride, b_batch_stride};
+  const int64_t sls[4] = {a_limb_stride, a_limb_stride, b_limb_stride, b_limb_stride};
+  for (int c = 0; c < 4; ++c) {
+    int st = ntt::forward_columns_internal(workspace + c * comp, srcs[c], batch, limbs, static_cast<int>(log_n),
+                                           limbs * n, n, sbs[c], sls[c], rows, forward_twiddles,
+                                           forward_twiddles_shoup, row_start, true, s);
+    if (st != kOk) return st;
+  }
+  HmArgs h;
+  h.ws = workspace;
+  h.out = out;
+  h.batch = batch;
+  h.limbs = limbs;
+  h.row_start = row_start;
+  h.log_n = static_cast<int>(log_n);
+  h.params = ParamView{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  h.rows = rows;
+  h.fwd_w = forward_twiddles;
+  h.fwd_wp = forward_twiddles_shoup;
+  h.inv_w = inverse_twiddles;
+  h.inv_wp = inverse_twiddles_shoup;
+  const int64_t groups = n / (kTile * kMacTilesPerBlock);
+  FH_CHECK_ARG(groups * limbs * batch <= 0x7fffffff, "fused tensor product grid too large");
+  if (small_rows > 0) {
+    hipLaunchKernelGGL(k_hmult_tiles<true>, dim3(static_cast<uint32_t>(groups * small_rows * batch)),
+                       dim3(kBlock), 0, s, h, groups, int64_t{0}, small_rows);
+    FH_CHECK_LAUNCH("fused tensor product tiles (binary64 rows)");
+  }
+  if (small_rows < limbs) {
+    hipLaunchKernelGGL(k_hmult_tiles<false>, dim3(static_cast<uint32_t>(groups * (limbs - small_rows) * batch)),
+                       dim3(kBlock), 0, s, h, groups, small_rows, limbs - small_rows);
+    FH_CHECK_LAUNCH("fused tensor product tiles (integer rows)");
+  }
+  return ntt::inverse_columns_internal(out, 3 * batch, limbs, static_cast<int>(log_n), limbs * n, n, rows,
+                                       inverse_twiddles, inverse_twiddles_shoup, row_start, kOutStdMac, s);
 }

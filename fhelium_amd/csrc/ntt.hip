@@ -625,6 +625,26 @@ int inverse_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n, int
   return launch(a, batch * limbs, false, kFast, stream);
 }
 
+int forward_columns_internal(int64_t* dst, const int64_t* src, int64_t batch, int64_t limbs, int log_n,
+                             int64_t dst_batch_stride, int64_t dst_limb_stride, int64_t src_batch_stride,
+                             int64_t src_limb_stride, const NttRow* rows, const int64_t* tw,
+                             const int64_t* twp, int64_t row_start, bool to_montgomery, hipStream_t stream) {
+  FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "column-only forward needs logN 12..16");
+  NttArgs a{dst, dst_batch_stride, dst_limb_stride, limbs, 0, rows, tw, twp, row_start, log_n,
+            to_montgomery ? 1 : 0, 0, src, src_batch_stride, src_limb_stride};
+  const int s1 = log_n - kTileLog;
+  const int T = (1 << s1) / 16;
+  const int C = kBlock / T;
+  const int64_t total_rows = batch * limbs;
+  for (int64_t start = 0; start < total_rows; start += kMaxGridY) {
+    const int64_t rows_now = total_rows - start < kMaxGridY ? total_rows - start : kMaxGridY;
+    a.row_offset = start;
+    launch_columns<kFast>(true, s1, dim3(kTile / C, static_cast<uint32_t>(rows_now)), stream, a);
+    FH_CHECK_LAUNCH("ntt forward column pass");
+  }
+  return kOk;
+}
+
 int inverse_columns_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n,
                              int64_t batch_stride, int64_t limb_stride, const NttRow* rows,
                              const int64_t* tw, const int64_t* twp, int64_t row_start,

@@ -204,6 +204,12 @@ struct U64Path {
 int inverse_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n, int64_t batch_stride,
                      int64_t limb_stride, const NttRow* rows, const int64_t* tw, const int64_t* twp,
                      int64_t row_start, int output_form, bool in_mid, hipStream_t stream);
+// Only the forward column pass (logN 12..16, fast arithmetic) from src rows
+// into dst rows, leaving working-format values for a fused tile pass.
+int forward_columns_internal(int64_t* dst, const int64_t* src, int64_t batch, int64_t limbs, int log_n,
+                             int64_t dst_batch_stride, int64_t dst_limb_stride, int64_t src_batch_stride,
+                             int64_t src_limb_stride, const NttRow* rows, const int64_t* tw,
+                             const int64_t* twp, int64_t row_start, bool to_montgomery, hipStream_t stream);
 // Only the column pass of that inverse (logN 12..16): for rows whose tile
 // stages were already applied, holding working-format values.
 int inverse_columns_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n,

@@ -523,6 +523,27 @@ class CkksEngine:
 
     # convenience: HMult as the reference's examples compose it (README.md:66-69)
     def hmult(self, a: Ciphertext, b: Ciphertext) -> Ciphertext:
-        return self.rescale_to_next_level(
-            self.relinearize(self.multiply(self.coefficient_domain_to_ntt_domain(a),
-                                           self.coefficient_domain_to_ntt_domain(b))))
+        """rescale(relinearize(multiply(NTT(a), NTT(b)))).  With the fused
+        key switch and fast NTT arithmetic the first three steps run as one
+        fused tensor product (fh_ckks_hmult_tensor_coefficient) feeding the
+        key switch; the result is bit-identical to the composition."""
+
+        from fhelium_amd.ntt import MODE_FAST
+
+        if not (self.fused_keyswitch and self.runtime.ntt._mode == MODE_FAST and self.config.logN >= 12):
+            return self.rescale_to_next_level(
+                self.relinearize(self.multiply(self.coefficient_domain_to_ntt_domain(a),
+                                               self.coefficient_domain_to_ntt_domain(b))))
+        for ct in (a, b):
+            ct.assert_state(polynomial_domain="coefficient", residue_representation="standard",
+                            modulus_basis="Q", components=2)
+        self._same_layout(a, b)
+        scale = a.scale * b.scale
+        if not math.isfinite(scale) or scale <= 0:
+            raise InvalidScaleError("product scale is not positive and finite")
+        key = self.relinearization_key
+        if key is None:
+            key = self.generate_relinearization_key()
+        d = self.key_switcher.tensor_product_coefficient(a.data, b.data, a.level)
+        out = self.key_switcher.key_switch_fused(d[2], key, a.level, base0=d[0], base1=d[1])
+        return self.rescale_to_next_level(Ciphertext(out, a.level, scale, "coefficient", "standard", "Q"))

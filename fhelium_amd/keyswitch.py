@@ -175,6 +175,42 @@ class HybridKeySwitcher:
         # same stream, which is ordered after these launches.
         return out.reshape(2, *batch_shape, lq, n)
 
+    def tensor_product_coefficient(self, a: torch.Tensor, b: torch.Tensor, level: int) -> torch.Tensor:
+        """[3, *batch, Q_l, N] coefficient-domain d0, d1, d2 of two
+        coefficient-domain ciphertexts a, b ([2, *batch, Q_l, N], canonical
+        standard): NTT of both, tensor product and inverse NTT in one fused
+        pass (fh_ckks_hmult_tensor_coefficient, fast arithmetic)."""
+
+        from fhelium_amd import _native as nat
+
+        rt = self.rt
+        cfg = self.config
+        n = cfg.N
+        start = rt.row_start(level)
+        lq = cfg.num_q_primes - start
+        if a.shape != b.shape or a.dim() < 3 or a.size(0) != 2 or a.shape[-2:] != (lq, n):
+            raise ValueError("tensor product operands must be [2, *batch, Q_level, N] of equal shape")
+        batch_shape = a.shape[1:-2]
+        av = a.reshape(2, -1, lq, n)
+        bv = b.reshape(2, -1, lq, n)
+        for t in (av, bv):
+            if t.stride(-1) != 1:
+                raise ValueError("tensor product operands need unit coefficient stride")
+        batch = av.size(1)
+        small = [q < SMALL_PRIME_LIMIT for q in cfg.moduli[start: start + lq]]
+        small_rows = sum(small)
+        if any(small[small_rows:]):
+            raise ValueError("fused tensor product needs the binary64 rows before the integer rows")
+        ws = torch.empty((4, batch, lq, n), dtype=torch.int64, device=a.device)
+        out = torch.empty((3, batch, lq, n), dtype=torch.int64, device=a.device)
+        nt = rt.ntt_tables
+        nat.call("fh_ckks_hmult_tensor_coefficient", av.data_ptr(), bv.data_ptr(), av.stride(0), av.stride(1),
+                 av.stride(2), bv.stride(0), bv.stride(1), bv.stride(2), batch, lq, cfg.logN, start, small_rows,
+                 nat.t2(rt.rns_params), nt.rows.data_ptr(), nt.fwd.data_ptr(), nt.fwd_shoup.data_ptr(),
+                 nt.inv.data_ptr(), nt.inv_shoup.data_ptr(), ws.data_ptr(), out.data_ptr(),
+                 nat.stream_ptr(a.device))
+        return out.reshape(3, *batch_shape, lq, n)
+
     def algorithmic_bytes(self, batch: int, lq: int, lqp: int, dnum: int, per_entry_keys: bool,
                           bases: int) -> int:
         """Compulsory HBM bytes of one fused key switch: the component and

@@ -209,6 +209,29 @@ int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lq
                        const int64_t* galois_sign_tables, int64_t* workspace_digits,
                        int64_t* workspace_inter, int64_t* workspace_acc, fh_view3 base0,
                        fh_view3 base1, fh_view3 out0, fh_view3 out1, int32_t mode, void* stream);
+
+/* HMult front of the batching tier: d0 = a0 b0, d1 = a0 b1 + a1 b0,
+ * d2 = a1 b1 of two coefficient-domain ciphertexts, returned in the
+ * coefficient domain with canonical standard residues -- the composition
+ * coefficient_domain_to_ntt_domain (ckks_engine.py:876-903) of both inputs,
+ * multiply (:1530-1606) and the inverse NTT that starts relinearize
+ * (:1608-1640), bit for bit, in six launches (four forward column passes,
+ * one fused tile pass per arithmetic path, one inverse column pass).
+ *   a, b             [2, batch, limbs, N] canonical standard rows at level
+ *                    row_start (component, batch, limb strides given;
+ *                    coefficient stride 1)
+ *   small_rows       leading rows whose prime is below 2^46
+ *   workspace        [4, batch, limbs, N] contiguous
+ *   out              [3, batch, limbs, N] contiguous
+ * Fast NTT arithmetic only (the engine's default mode). */
+int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, int64_t a_comp_stride,
+                                     int64_t a_batch_stride, int64_t a_limb_stride, int64_t b_comp_stride,
+                                     int64_t b_batch_stride, int64_t b_limb_stride, int64_t batch,
+                                     int64_t limbs, int64_t log_n, int64_t row_start, int64_t small_rows,
+                                     fh_table2 rns_params, const void* ntt_rows,
+                                     const int64_t* forward_twiddles, const int64_t* forward_twiddles_shoup,
+                                     const int64_t* inverse_twiddles, const int64_t* inverse_twiddles_shoup,
+                                     int64_t* workspace, int64_t* out, void* stream);
 /* keyswitch_moddown_qp_to_q */
 int fh_ckks_moddown(fh_view3 out, fh_view3 q_residues, fh_view3 p_residues,
                     fh_table2 p_drop_inverses, fh_table2 rns_params, int64_t batch,

@@ -1,0 +1,56 @@
+"""The fused HMult front (fh_ckks_hmult_tensor_coefficient) against the
+composition it replaces: coefficient_domain_to_ntt_domain of both inputs,
+multiply, and the inverse NTT that starts relinearize
+(ckks_engine.py:876-903, 1530-1606, 1608-1640), bit for bit; and the whole
+fused hmult against rescale(relinearize(multiply(...)))."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from fhelium_amd.engine import CkksEngine
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=[12, 16])
+def eng(request):
+    e = CkksEngine(H.config(request.param), seed=61)
+    e.generate_secret_key()
+    e.generate_relinearization_key()
+    return e
+
+
+def composed(eng, a, b):
+    return eng.multiply(eng.coefficient_domain_to_ntt_domain(a), eng.coefficient_domain_to_ntt_domain(b))
+
+
+@pytest.mark.parametrize("level,batch", [(0, (3,)), (17, ()), (32, (2,))])
+def test_tensor_product_matches_composition(eng, level, batch):
+    rng = np.random.default_rng(level)
+    x = rng.uniform(-1, 1, batch + (eng.config.slots,))
+    y = rng.uniform(-1, 1, batch + (eng.config.slots,))
+    a, b = eng.encrypt(x, level=level), eng.encrypt(y, level=level)
+    want = eng.runtime.ntt.inverse_to_standard(composed(eng, a, b).data, eng.runtime.row_start(level))
+    got = eng.key_switcher.tensor_product_coefficient(a.data, b.data, level)
+    assert torch.equal(got, want)
+    fused = eng.hmult(a, b)
+    ref = eng.rescale_to_next_level(eng.relinearize(composed(eng, a, b)))
+    assert torch.equal(fused.data, ref.data)
+    assert fused.scale == ref.scale and fused.level == ref.level == level + 1
+    assert np.abs(eng.decrypt(fused).real - x * y).max() < 2e-5
+
+
+def test_tensor_product_on_strided_inputs(eng):
+    x = np.random.default_rng(5).uniform(-1, 1, (4, eng.config.slots))
+    ct = eng.encrypt(x, level=5)
+    a = ct.data[:, ::2]  # batch-strided views
+    b = ct.data[:, 1::2]
+    want = eng.runtime.ntt.inverse_to_standard(
+        eng.multiply(eng.coefficient_domain_to_ntt_domain(type(ct)(a.contiguous(), 5, ct.scale)),
+                     eng.coefficient_domain_to_ntt_domain(type(ct)(b.contiguous(), 5, ct.scale))).data,
+        eng.runtime.row_start(5))
+    assert torch.equal(eng.key_switcher.tensor_product_coefficient(a, b, 5), want)
