@@ -72,8 +72,8 @@ struct KsDigit {
   const int64_t* basis_std;    // [D-1, QP0] M_r R mod p_j
   int64_t f64_ok;              // every source prime < 2^45: digits stored as binary64
   const int64_t* basis_shoup;  // [D-1, QP0] Shoup companions of basis_std
-  const double* moddown_f64;   // P digit only: [2P+1, Q0] binary64 ModDown table
-  int64_t moddown_stride;      // its row stride (Q0)
+  const double* moddown_f64;   // P digit only: [Q0, 16] binary64 ModDown records
+  int64_t moddown_stride;      // record pitch (16 words)
 };
 static_assert(sizeof(KsDigit) == 16 * 8, "KsDigit must stay 16 words");
 
@@ -912,7 +912,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     dlo[r] = static_cast<double>(static_cast<uint32_t>(dg[r] & ((int64_t{1} << 30) - 1)));
     dhi[r] = static_cast<double>(static_cast<uint32_t>(dg[r] >> 30));
   }
-  const int64_t q0 = a.pdig.moddown_stride;
+  const int64_t pitch = a.pdig.moddown_stride;
   // Final canonical addition of the (possibly gathered) base, as
   // apply_coefficient_galois_automorphism + add_canonical compute it.
   auto finish = [&](int rr, int64_t row, int64_t value, int64_t twice_q) {
@@ -933,23 +933,32 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     const int64_t row = row_begin + rr;
     if (row >= a.lq) break;
     const int64_t grow = a.level + row;
-    const NttRow& nr = a.rows[grow];
-    if (!nr.small) continue;
+    // The row's 16-word record: M_r, 2^30 M_r, P^{-1}, q, 1/q (q = 0: an
+    // integer-path row).
+    Global<double>* rec = global(a.pdig.moddown_f64) + grow * pitch;
+    double t[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) t[w] = rec[w];
+    const double q = t[14], qinv = t[15];
+    if (q == 0.0) continue;
     // (x - X_P) P^{-1} mod q with X_P mod q = sum_r c_lo M_r + c_hi 2^30 M_r,
     // all in signed binary64 (|sum| < 4.2q), canonicalised once.
-    const double q = nr.qd, qinv = nr.qinv;
-    Global<double>* tab = global(a.pdig.moddown_f64) + grow;
     double xp = 0.0;
+    double pinv = 0.0;
 #pragma unroll
-    for (int r = 0; r < kMaxP; ++r) {
-      if (r >= p_count) break;
-      xp += f64_mulmod_signed(tab[r * q0], dlo[r], q, qinv);
-      xp += f64_mulmod_signed(tab[(p_count + r) * q0], dhi[r], q, qinv);
+    for (int r = 0; r < 6; ++r) {
+      if (r == p_count) pinv = t[2 * r];
+      if (r >= p_count) continue;
+      xp += f64_mulmod_signed(t[r], dlo[r], q, qinv);
+#pragma unroll
+      for (int h = 0; h < 6; ++h)
+        if (h == p_count) xp += f64_mulmod_signed(t[h + r], dhi[r], q, qinv);
     }
+    if (p_count == 6) pinv = t[12];
     const double diff = u64_to_f64(static_cast<uint64_t>(xv[rr])) - xp;
-    double res = f64_mulmod_signed(tab[2 * p_count * q0], diff, q, qinv);
+    double res = f64_mulmod_signed(pinv, diff, q, qinv);
     res = res < 0.0 ? res + q : res;
-    finish(rr, row, static_cast<int64_t>(f64_to_u64(res)), static_cast<int64_t>(nr.twice_q));
+    finish(rr, row, static_cast<int64_t>(f64_to_u64(res)), 2 * static_cast<int64_t>(q));
   }
   // Rows of primes above 2^46 (the base prime): the integer formula.
   for (int rr = 0; rr < kRows; ++rr) {
@@ -1031,7 +1040,7 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "fused key switch supports logN 12..16");
   FH_CHECK_ARG(batch >= 1 && batch <= 32767, "fused key switch batch %lld out of range",
                (long long)batch);
-  FH_CHECK_ARG(lq >= 1 && lqp > lq && lqp - lq <= kMaxP, "bad Q/QP row counts");
+  FH_CHECK_ARG(lq >= 1 && lqp > lq && lqp - lq <= 6, "bad Q/QP row counts (at most 6 P primes)");
   FH_CHECK_ARG(digit_count >= 1 && digit_count <= 16 && digit_count * batch <= 65535, "bad digit count");
   FH_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (exact) or 1 (fast)");
   FH_CHECK_ARG(component.data && digit_table && p_digit_table && p_inverse_q && rns_params.data &&

@@ -175,26 +175,36 @@ def _as_signed64(v: int) -> int:
 MODDOWN_SPLIT_BITS = 30
 
 
+MODDOWN_RECORD_WORDS = 16
+
+
 def moddown_binary64_table(moduli: tuple[int, ...], p_ids: tuple[int, ...],
                            q_ids: tuple[int, ...]) -> torch.Tensor:
-    """[2P + 1, Q0] float64 rows for ModDown on Q rows below 2^46:
-    M_r mod q, 2^30 M_r mod q (M_r = prod_{t<r} p_t), then P^{-1} mod q.
-    Rows of larger primes hold zeros (they take the integer path)."""
+    """[Q0, 16] float64 ModDown records, one per Q row whose prime is below
+    2^46: words 0..P-1 hold M_r mod q, words P..2P-1 hold 2^30 M_r mod q
+    (M_r = prod_{t<r} p_t), word 2P holds P^{-1} mod q, word 14 holds q and
+    word 15 holds 1/q.  Rows of larger primes are all zero (they take the
+    integer path).  One 128-byte record per row lets a wave fetch a row's
+    constants with two scalar loads."""
 
     p = [moduli[i] for i in p_ids]
+    if 2 * len(p) + 1 > 14:
+        raise ValueError("the ModDown record holds at most 6 P primes")
     prefix = [1]
     for v in p[:-1]:
         prefix.append(prefix[-1] * v)
     p_product = math.prod(p)
-    out = torch.zeros((2 * len(p) + 1, len(q_ids)), dtype=torch.float64)
+    out = torch.zeros((len(q_ids), MODDOWN_RECORD_WORDS), dtype=torch.float64)
     for j, qi in enumerate(q_ids):
         q = moduli[qi]
         if q >= (1 << 46):
             continue
         for r, m in enumerate(prefix):
-            out[r, j] = float(m % q)
-            out[len(p) + r, j] = float((m << MODDOWN_SPLIT_BITS) % q)
-        out[2 * len(p), j] = float(pow(p_product, -1, q))
+            out[j, r] = float(m % q)
+            out[j, len(p) + r] = float((m << MODDOWN_SPLIT_BITS) % q)
+        out[j, 2 * len(p)] = float(pow(p_product, -1, q))
+        out[j, 14] = float(q)
+        out[j, 15] = 1.0 / q
     return out
 
 
