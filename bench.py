@@ -7,7 +7,9 @@ l = 34 - d, d ~ U{2..34} (so every rescale is legal, l in 0..32), slots
 uniform[-1, 1] at scale 2^40; each pair does HMult (NTT conversion of both
 inputs, multiply, relinearize, rescale) and then a rotation by its own step
 s ~ U{1..1024} with that step's direct key (all keys HBM-resident).
-  * plain:   one ciphertext per call sequence;
+  * plain:   the drop-in per-op path (the reference's op sequence, one
+             launch per op), one ciphertext per call sequence;
+  * unbatched fused: the fused kernels, one ciphertext per call sequence;
   * batched: ciphertexts of equal level processed as one batched call
              (shared relinearization key, per-entry rotation keys).
 One "step" is one pass over all C pairs; value = (HMults + rotations) per
@@ -37,10 +39,10 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) per algorithmic
 # byte of the fused key switch on this workload; the rocprofv3 --pmc passes
 # and the arithmetic are recorded in profiles/keyswitch_fused_traffic.md.
-KS_TRAFFIC_BYTES_PER_ALG_BYTE = 3.579
+KS_TRAFFIC_BYTES_PER_ALG_BYTE = 3.578
 # Measured VALU wave-instructions (SQ_INSTS_VALU) per algorithmic byte of the
 # fused key switch on this workload (profiles/keyswitch_fused_valu_issue.md).
-KS_VALU_INSTS_PER_ALG_BYTE = 0.37768
+KS_VALU_INSTS_PER_ALG_BYTE = 0.38405
 # gfx950 issues one wave64 VALU instruction per CU per cycle for full-rate
 # operations (the rate behind the 78.6 TFLOP/s binary64 FMA peak):
 # 256 CUs x 2.4 GHz.
@@ -203,6 +205,41 @@ def cpu_baseline(eng, seconds_budget: float = 30.0):
                       "logN=16 L34, oracle C restatement with OpenMP over RNS rows"}
 
 
+def ntt_measure(eng, reps: int = 5) -> dict:
+    """Forward (to Montgomery) and inverse (to standard) NTT of the ring-kernel
+    shape [64 ciphertexts, 2 polys, 35 limbs, 2^16], uniform residues,
+    out of place; algorithmic bytes = one read and one write of every limb."""
+
+    cfg = eng.config
+    q = torch.tensor(cfg.moduli[: cfg.num_q_primes], dtype=torch.int64, device="cuda").view(1, 1, -1, 1)
+    x = torch.randint(0, 2**62, (64, 2, cfg.num_q_primes, cfg.N), dtype=torch.int64, device="cuda") % q
+    ntt = eng.runtime.ntt
+    limb_bytes = 2 * cfg.N * 8
+    nbytes = x.numel() // cfg.N * limb_bytes
+    out = {}
+    for name, fn in (("forward", lambda t: ntt.forward_to_montgomery(t, 0)),
+                     ("inverse", lambda t: ntt.inverse_to_standard(t, 0))):
+        y = fn(x)  # warm-up
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+        for i in range(reps):
+            evs[2 * i].record()
+            y = fn(x)
+            evs[2 * i + 1].record()
+        torch.cuda.synchronize()
+        ms = float(np.median([evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(reps)]))
+        out[name] = {"ms": ms, "achieved": nbytes / (ms * 1e-3) / 1e9}
+        del y
+    del x
+    achieved = 2 * nbytes / ((out["forward"]["ms"] + out["inverse"]["ms"]) * 1e-3) / 1e9
+    return {
+        "bound": "hbm", "kernel": "fh_ntt_forward / fh_ntt_inverse (two-pass radix-16, fast arithmetic)",
+        "workload": "[64, 2, 35, 65536] int64 residues, forward to Montgomery + inverse to standard",
+        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+        "algorithmic_bytes_per_call": nbytes,
+        "forward_ms": out["forward"]["ms"], "inverse_ms": out["inverse"]["ms"],
+    }
+
+
 def serving_run(eng, args) -> dict:
     """Serving loop: Poisson arrivals of degree-7 Chebyshev (sigmoid-like,
     seeded) evaluations on uniform[-4, 4] inputs plus a 64-slot window sum,
@@ -291,9 +328,17 @@ def main() -> int:
 
     if args.skip_plain:
         ms_plain, plh, plr = float("nan"), [], []
+        ms_unb, ubh, ubr = float("nan"), [], []
     else:
+        # plain: the drop-in path -- the reference's op sequence on the
+        # per-op kernels (one launch per op), one ciphertext per call
+        eng.fused_keyswitch = False
         ms_plain, plh, plr = timed(lambda h, r: run_plain(eng, groups, h, r), args.steps, args.warmup)
-    log(f"plain   {ms_plain:.2f} ms/step")
+        eng.fused_keyswitch = True
+        # unbatched fused: the fused kernels, still one ciphertext per call
+        ms_unb, ubh, ubr = timed(lambda h, r: run_plain(eng, groups, h, r), args.steps, args.warmup)
+    log(f"plain   {ms_plain:.2f} ms/step (per-op drop-in path)")
+    log(f"unbatched fused {ms_unb:.2f} ms/step")
     if args.calibration_copy:
         src = torch.randint(0, 1 << 40, (1 << 25,), dtype=torch.int64, device="cuda")
         dst = torch.empty_like(src)
@@ -303,14 +348,11 @@ def main() -> int:
         log("calibration copy: 268435456 B read + 268435456 B written (torch.neg, int64 x 2^25)")
         del src, dst
     def start_recorders():
-        eng.runtime.ntt.recorder = []
         eng.key_switcher.recorder = []
 
     ms_batch, bh, br = timed(lambda h, r: run_batched(eng, groups, h, r), args.steps, args.warmup,
                              start_recorders)
-    rec = eng.runtime.ntt.recorder
     ks_rec = eng.key_switcher.recorder
-    eng.runtime.ntt.recorder = None
     eng.key_switcher.recorder = None
     log(f"batched {ms_batch:.2f} ms/step")
 
@@ -319,10 +361,7 @@ def main() -> int:
     plain_r = [a.elapsed_time(b) for a, b in plr]
     batch_h = [a.elapsed_time(b) for a, b, _ in bh]
     batch_r = [a.elapsed_time(b) for a, b, _ in br]
-    ntt_ms = sum(a.elapsed_time(b) for a, b, _ in rec)
-    ntt_bytes = sum(nb for _, _, nb in rec)
-    ntt_calls = len(rec) // args.steps if args.steps else 0
-    ntt_achieved = ntt_bytes / (ntt_ms * 1e-3) / 1e9 if ntt_ms > 0 else None
+    ntt = ntt_measure(eng)
     # timed region only: the recorders start after the warmup steps
     ks_ms = sum(a.elapsed_time(b) for a, b, _ in ks_rec)
     ks_bytes = sum(nb for _, _, nb in ks_rec)
@@ -352,6 +391,8 @@ def main() -> int:
         "hmult_per_s": args.count / (sum(batch_h) / args.steps * 1e-3),
         "rotations_per_s": args.count / (sum(batch_r) / args.steps * 1e-3),
         "plain": None if args.skip_plain else {
+            "path": "drop-in per-op path: the reference's op sequence (one launch per op, ~100 per "
+                    "key switch), one ciphertext per call",
             "value": n_ops / (ms_plain * 1e-3),
             "ms_per_step": ms_plain,
             "hmult_per_s": args.count / (sum(plain_h) / args.steps * 1e-3),
@@ -359,7 +400,15 @@ def main() -> int:
             "hmult_p50_ms": percentile(plain_h, 50), "hmult_p99_ms": percentile(plain_h, 99),
             "rotation_p50_ms": percentile(plain_r, 50), "rotation_p99_ms": percentile(plain_r, 99),
         },
+        "unbatched_fused": None if args.skip_plain else {
+            "path": "fused kernels (8-launch key switch, fused HMult front), one ciphertext per call",
+            "value": n_ops / (ms_unb * 1e-3),
+            "ms_per_step": ms_unb,
+            "hmult_per_s": args.count / (sum(a.elapsed_time(b) for a, b in ubh) / args.steps * 1e-3),
+            "rotations_per_s": args.count / (sum(a.elapsed_time(b) for a, b in ubr) / args.steps * 1e-3),
+        },
         "batched_vs_plain": None if args.skip_plain else ms_plain / ms_batch,
+        "batched_vs_unbatched_fused": None if args.skip_plain else ms_unb / ms_batch,
         "p50_ms": percentile(batch_h + batch_r, 50),
         "p99_ms": percentile(batch_h + batch_r, 99),
         "latency_note": "batched p50/p99 are per batched call (one level group, HMult or rotation phase)",
@@ -388,13 +437,7 @@ def main() -> int:
                     if ks_ms > 0 else None,
             "valu_instructions_per_algorithmic_byte": KS_VALU_INSTS_PER_ALG_BYTE,
         },
-        "ntt_roofline": {
-            "bound": "hbm", "kernel": "fh_ntt_forward / fh_ntt_inverse (two-pass radix-16)",
-            "achieved": ntt_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (ntt_achieved / HBM_PEAK_GBS) if ntt_achieved else None,
-            "calls_per_step": ntt_calls,
-            "algorithmic_bytes_per_call": (ntt_bytes / len(rec)) if rec else None,
-        },
+        "ntt_roofline": ntt,
         "step_roofline": {
             "bound": "hbm", "algorithmic_bytes_per_step": step_bytes, "achieved": step_achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_achieved / HBM_PEAK_GBS,
