@@ -22,6 +22,7 @@ from fhelium_amd import codec
 from fhelium_amd import kernels as K
 from fhelium_amd import rns
 from fhelium_amd.config import CkksConfig, Preset
+from fhelium_amd.csprng import Csprng
 from fhelium_amd.core import (Ciphertext, CkksStateError, InvalidScaleError, KeySwitchKey,
                               MaximumLevelError, Plaintext, RelinearizationKey, RotationKey,
                               ScaleMismatchError, SecretKey)
@@ -43,9 +44,10 @@ class CkksEngine:
         # reference's per-digit op sequence.  Both are bit-identical.
         self.fused_keyswitch = fused_keyswitch
         self.allow_secret_key_generation = allow_secret_key_generation
-        # Seeded device sampler (Philox); key and ciphertext randomness never
-        # leaves HBM.
-        self._gen = torch.Generator(device=self.device).manual_seed(seed)
+        # Seeded ChaCha20 sampler on the device (fhelium_amd/csprng.py); key
+        # and ciphertext randomness never leaves HBM.
+        self.rng = Csprng(seed, self.device)
+        self._moduli_cache: dict[tuple[int, ...], torch.Tensor] = {}
         self.secret_key: SecretKey | None = None
         self.relinearization_key: RelinearizationKey | None = None
         self.rotation_keys: dict[int, RotationKey] = {}
@@ -55,19 +57,17 @@ class CkksEngine:
     # Sampling (this module's own seeded sampler)
     # ------------------------------------------------------------------
     def _uniform_qp(self, shape_prefix: tuple[int, ...], rows: tuple[int, ...]) -> torch.Tensor:
-        q = torch.tensor([self.config.moduli[i] for i in rows], dtype=torch.int64, device=self.device)
-        raw = torch.randint(0, 2**62, (*shape_prefix, len(rows), self.config.N), generator=self._gen,
-                            dtype=torch.int64, device=self.device)
-        return raw % q.view(*([1] * len(shape_prefix)), -1, 1)
+        q = self._moduli_cache.get(rows)
+        if q is None:
+            q = torch.tensor([self.config.moduli[i] for i in rows], dtype=torch.int64, device=self.device)
+            self._moduli_cache[rows] = q
+        return self.rng.uniform_mod(q, tuple(shape_prefix), self.config.N)
 
     def _gaussian(self, shape_prefix: tuple[int, ...]) -> torch.Tensor:
-        e = torch.randn((*shape_prefix, self.config.N), generator=self._gen, dtype=torch.float64,
-                        device=self.device)
-        return torch.round(e * self.config.sigma).to(torch.int64)
+        return self.rng.gaussian((*shape_prefix, self.config.N), self.config.sigma)
 
     def _ternary(self) -> torch.Tensor:
-        return torch.randint(-1, 2, (self.config.N,), generator=self._gen, dtype=torch.int64,
-                             device=self.device)
+        return self.rng.ternary((self.config.N,))
 
     def _lift_signed(self, coefficients: torch.Tensor, rows: tuple[int, ...]) -> torch.Tensor:
         """Signed integer coefficients [*, N] -> canonical residues [*, rows, N] on device."""

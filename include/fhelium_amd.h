@@ -232,6 +232,27 @@ int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, int64_t
                                      const int64_t* forward_twiddles, const int64_t* forward_twiddles_shoup,
                                      const int64_t* inverse_twiddles, const int64_t* inverse_twiddles_shoup,
                                      int64_t* workspace, int64_t* out, void* stream);
+
+/* ---- Sampling: this backend's own ChaCha20 generator ------------------
+ * Replaces the reference's ChaCha20 streams (fhelium/rng/csprng.py:150-250,
+ * drawn from the external triton-csprng package): the RFC 8439 block
+ * function, one block per thread.  key: 8 host words, nonce: 3 host words;
+ * block b of a request uses block counter `counter + b` (the request must
+ * not wrap the 32-bit counter).  Device outputs. */
+int fh_chacha20_keystream(const uint32_t* key, const uint32_t* nonce, uint32_t counter, int64_t blocks,
+                          uint32_t* out, void* stream);
+/* out[e, r, i] uniform in [0, moduli[r]): 128 keystream bits per value
+ * reduced mod q (the reference's randint_channels, csprng.py:207-219). */
+int fh_sample_uniform_mod(const uint32_t* key, const uint32_t* nonce, uint32_t counter,
+                          const int64_t* moduli, int64_t outer, int64_t rows, int64_t count, int64_t* out,
+                          void* stream);
+/* out[g] uniform in {-1, 0, 1}: 64 keystream bits per value, mod 3, minus 1. */
+int fh_sample_ternary(const uint32_t* key, const uint32_t* nonce, uint32_t counter, int64_t total,
+                      int64_t* out, void* stream);
+/* out[g] = round(sigma * z), z standard normal by Box-Muller on two 53-bit
+ * uniforms (the reference's discrete_gaussian, csprng.py:221-233). */
+int fh_sample_gaussian(const uint32_t* key, const uint32_t* nonce, uint32_t counter, int64_t total,
+                       double sigma, int64_t* out, void* stream);
 /* keyswitch_moddown_qp_to_q */
 int fh_ckks_moddown(fh_view3 out, fh_view3 q_residues, fh_view3 p_residues,
                     fh_table2 p_drop_inverses, fh_table2 rns_params, int64_t batch,
