@@ -85,6 +85,8 @@ SIGNATURES: dict[str, list] = {
     "fh_sample_uniform_mod": [_ptr, _ptr, ctypes.c_uint32, _ptr, _i64, _i64, _i64, _ptr, _ptr],
     "fh_sample_ternary": [_ptr, _ptr, ctypes.c_uint32, _i64, _ptr, _ptr],
     "fh_sample_gaussian": [_ptr, _ptr, ctypes.c_uint32, _i64, ctypes.c_double, _ptr, _ptr],
+    "fh_ckks_encode": [_ptr, _i64, _i64, _ptr, ctypes.c_double, _ptr, _i64, _ptr, _ptr, _ptr],
+    "fh_ckks_decode": [_ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, ctypes.c_double, _ptr, _ptr, _ptr],
     "fh_ckks_hmult_tensor_coefficient": [_ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
                                          _i64, _i64, fh_table2, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
                                          _ptr],

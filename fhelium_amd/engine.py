@@ -258,6 +258,72 @@ class CkksEngine:
             K.to_montgomery_(data, self.runtime.params(level))
         return Plaintext(data=data, level=level, scale=scale, polynomial_domain=polynomial_domain)
 
+    def _slot_positions(self) -> torch.Tensor:
+        pos = self.__dict__.get("_slot_pos_dev")
+        if pos is None:
+            pos = torch.from_numpy(codec._slot_positions(self.config.N).astype(np.int32)).to(self.device)
+            self._slot_pos_dev = pos
+        return pos
+
+    def encode_on_device(self, values: torch.Tensor, *, level: int = 0, scale: float | None = None,
+                         polynomial_domain: str = "coefficient") -> Plaintext:
+        """``encode`` with the embedding on the device (fh_ckks_encode):
+        values [*batch, <=S] real or complex on the engine's device."""
+
+        from fhelium_amd import _native as nat
+
+        cfg = self.config
+        self.layout.check_level(level)
+        if polynomial_domain not in ("coefficient", "ntt"):
+            raise ValueError("polynomial_domain must be 'coefficient' or 'ntt'")
+        scale = cfg.default_scale if scale is None else float(scale)
+        if not (math.isfinite(scale) and scale > 0):
+            raise InvalidScaleError(f"plaintext scale must be positive and finite, got {scale}")
+        v = torch.as_tensor(values, device=self.device)
+        if v.dim() < 1 or v.size(-1) > cfg.slots:
+            raise ValueError(f"at most {cfg.slots} slots per plaintext")
+        batch_shape = tuple(v.shape[:-1])
+        flat = torch.zeros((int(np.prod(batch_shape)) if batch_shape else 1, cfg.slots), dtype=torch.complex128,
+                           device=self.device)
+        flat[:, : v.size(-1)] = v.reshape(-1, v.size(-1)).to(torch.complex128)
+        ws = torch.empty((flat.size(0), cfg.N, 2), dtype=torch.float64, device=self.device)
+        rows = self.layout.prime_ids(level)
+        moduli = self._moduli_cache.get(rows)
+        if moduli is None:
+            moduli = torch.tensor([cfg.moduli[i] for i in rows], dtype=torch.int64, device=self.device)
+            self._moduli_cache[rows] = moduli
+        data = torch.empty((flat.size(0), len(rows), cfg.N), dtype=torch.int64, device=self.device)
+        nat.call("fh_ckks_encode", torch.view_as_real(flat).data_ptr(), flat.size(0), cfg.logN,
+                 self._slot_positions().data_ptr(), scale, moduli.data_ptr(), len(rows), ws.data_ptr(),
+                 data.data_ptr(), nat.stream_ptr(self.device))
+        data = data.reshape(*batch_shape, len(rows), cfg.N)
+        if polynomial_domain == "ntt":
+            self.runtime.ntt.forward_to_montgomery_(data, level)
+        else:
+            K.to_montgomery_(data, self.runtime.params(level))
+        return Plaintext(data=data, level=level, scale=scale, polynomial_domain=polynomial_domain)
+
+    def decrypt_on_device(self, ct: Ciphertext) -> torch.Tensor:
+        """Slots [*batch, S] complex128 on the device: decryption, then the
+        centered CRT over the last two Q rows and the embedding on the
+        device (fh_ckks_decode)."""
+
+        from fhelium_amd import _native as nat
+
+        cfg = self.config
+        res = self.decrypt_residues(ct)
+        lq = res.size(-2)
+        if lq < 2:
+            raise ValueError("device decode needs at least two Q rows")
+        flat = res.reshape(-1, lq, cfg.N)
+        qa, qb = cfg.moduli[ct.level + lq - 1], cfg.moduli[ct.level + lq - 2]
+        ws = torch.empty((flat.size(0), cfg.N, 2), dtype=torch.float64, device=self.device)
+        out = torch.empty((flat.size(0), cfg.slots), dtype=torch.complex128, device=self.device)
+        nat.call("fh_ckks_decode", flat[:, lq - 1].data_ptr(), flat[:, lq - 2].data_ptr(), flat.stride(0),
+                 flat.size(0), cfg.logN, qa, qb, pow(qa, -1, qb), self._slot_positions().data_ptr(), float(ct.scale),
+                 ws.data_ptr(), torch.view_as_real(out).data_ptr(), nat.stream_ptr(self.device))
+        return out.reshape(*res.shape[:-2], cfg.slots)
+
     def _check_plaintext(self, ct: Ciphertext, pt: Plaintext, domain: str) -> None:
         if pt.level != ct.level:
             raise ValueError(f"Plaintext level {pt.level} does not match ciphertext level {ct.level}.")

@@ -253,6 +253,26 @@ int fh_sample_ternary(const uint32_t* key, const uint32_t* nonce, uint32_t count
  * uniforms (the reference's discrete_gaussian, csprng.py:221-233). */
 int fh_sample_gaussian(const uint32_t* key, const uint32_t* nonce, uint32_t counter, int64_t total,
                        double sigma, int64_t* out, void* stream);
+
+/* ---- Slot codec on the device (binary64) --------------------------------
+ * The reference encodes and decodes on the host side of
+ * ckks_plaintext_codec.py:181-316 / ckks_decryptor.py:96-272 through
+ * slot_embedding.py; these run the same canonical embedding next to the
+ * data.  slot_positions: int32 [N/2], (3^{-j} mod 2N - 1) / 2.
+ * fh_ckks_encode: slots [batch, N/2] complex as (re, im) binary64 pairs ->
+ *   out [batch, limbs, N] canonical standard residues of
+ *   rint(scale * coefficient) for moduli[0..limbs); workspace: batch * N
+ *   complex (16 bytes each). */
+int fh_ckks_encode(const double* slots, int64_t batch, int64_t log_n, const int32_t* slot_positions,
+                   double scale, const int64_t* moduli, int64_t limbs, double* workspace, int64_t* out,
+                   void* stream);
+/* fh_ckks_decode: centered CRT of two canonical residue rows (batch stride
+ * batch_stride) over moduli a, b (a^{-1} mod b given), then the embedding
+ * divided by scale -> slots [batch, N/2] complex (re, im) binary64. */
+int fh_ckks_decode(const int64_t* residues_a, const int64_t* residues_b, int64_t batch_stride, int64_t batch,
+                   int64_t log_n, int64_t modulus_a, int64_t modulus_b, int64_t modulus_a_inverse_mod_b,
+                   const int32_t* slot_positions, double scale, double* workspace, double* slots,
+                   void* stream);
 /* keyswitch_moddown_qp_to_q */
 int fh_ckks_moddown(fh_view3 out, fh_view3 q_residues, fh_view3 p_residues,
                     fh_table2 p_drop_inverses, fh_table2 rns_params, int64_t batch,
