@@ -862,6 +862,12 @@ struct ModDownArgs {
   int has_base0, has_base1;
   const int64_t* gal_idx;   // Galois gather of base0 (rotations) or null
   const int64_t* gal_sign;
+  // Optional fused rescale (null: none): rows 1..lq-1 of the result are
+  // written as rows 0..lq-2 of rescale_drop_leading_prime_nearest's output
+  // (ckks_cpu.cpp:204-275): ((c_i - c_0) q_0^{-1} + [c_0 > half_drop]) mod q_i.
+  const int64_t* rescale_inv;        // [lq - 1] q_0^{-1} mod q_i, standard
+  const int64_t* rescale_inv_shoup;  // [lq - 1] Shoup companions
+  int64_t half_drop;
 };
 
 __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
@@ -915,34 +921,22 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   const int64_t pitch = a.pdig.moddown_stride;
   // Final canonical addition of the (possibly gathered) base, as
   // apply_coefficient_galois_automorphism + add_canonical compute it.
-  auto finish = [&](int rr, int64_t row, int64_t value, int64_t twice_q) {
-    if (has_base) {
-      int64_t bval = bv[rr];
-      if (a.gal_idx != nullptr && c == 0) {
-        if (negate) bval = -bval;
-        bval = canonicalize(shift_positive(bval, twice_q), twice_q);
-      }
-      int64_t sum = bval + value;
-      if (sum >= twice_q) sum -= twice_q;
-      value = canonicalize(sum, twice_q);
+  auto with_base = [&](int64_t bval, int64_t value, int64_t twice_q) -> int64_t {
+    if (!has_base) return value;
+    if (a.gal_idx != nullptr && c == 0) {
+      if (negate) bval = -bval;
+      bval = canonicalize(shift_positive(bval, twice_q), twice_q);
     }
-    out.at(b, row, i) = value;
+    int64_t sum = bval + value;
+    if (sum >= twice_q) sum -= twice_q;
+    return canonicalize(sum, twice_q);
   };
-#pragma unroll
-  for (int rr = 0; rr < kRows; ++rr) {
-    const int64_t row = row_begin + rr;
-    if (row >= a.lq) break;
-    const int64_t grow = a.level + row;
-    // The row's 16-word record: M_r, 2^30 M_r, P^{-1}, q, 1/q (q = 0: an
-    // integer-path row).
-    Global<double>* rec = global(a.pdig.moddown_f64) + grow * pitch;
-    double t[16];
-#pragma unroll
-    for (int w = 0; w < 16; ++w) t[w] = rec[w];
+  // floor(X / P) mod q of a binary64 row from its 16-word record: M_r,
+  // 2^30 M_r, P^{-1}, q, 1/q -- (x - X_P) P^{-1} mod q with
+  // X_P mod q = sum_r c_lo M_r + c_hi 2^30 M_r, all in signed binary64
+  // (|sum| < 4.2q), canonicalised once.
+  auto moddown_f64 = [&](const double (&t)[16], int64_t xval) -> int64_t {
     const double q = t[14], qinv = t[15];
-    if (q == 0.0) continue;
-    // (x - X_P) P^{-1} mod q with X_P mod q = sum_r c_lo M_r + c_hi 2^30 M_r,
-    // all in signed binary64 (|sum| < 4.2q), canonicalised once.
     double xp = 0.0;
     double pinv = 0.0;
 #pragma unroll
@@ -955,10 +949,50 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
         if (h == p_count) xp += f64_mulmod_signed(t[h + r], dhi[r], q, qinv);
     }
     if (p_count == 6) pinv = t[12];
-    const double diff = u64_to_f64(static_cast<uint64_t>(xv[rr])) - xp;
+    const double diff = u64_to_f64(static_cast<uint64_t>(xval)) - xp;
     double res = f64_mulmod_signed(pinv, diff, q, qinv);
     res = res < 0.0 ? res + q : res;
-    finish(rr, row, static_cast<int64_t>(f64_to_u64(res)), 2 * static_cast<int64_t>(q));
+    return static_cast<int64_t>(f64_to_u64(res));
+  };
+  auto load_record = [&](int64_t grow, double (&t)[16]) {
+    Global<double>* rec = global(a.pdig.moddown_f64) + grow * pitch;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) t[w] = rec[w];
+  };
+  // Fused rescale: every thread needs the final row-0 value of its
+  // coefficient (row 0 is a binary64 row: the dropped prime is a scale prime).
+  const bool rescale = a.rescale_inv != nullptr;
+  int64_t c0 = 0;
+  int64_t round_up = 0;
+  if (rescale) {
+    double t0[16];
+    load_record(a.level, t0);
+    const int64_t x0 = row_begin == 0 ? xv[0] : x[0];
+    const int64_t b0 = row_begin == 0 ? bv[0] : (has_base ? base.at(b, 0, src_i) : 0);
+    c0 = with_base(b0, moddown_f64(t0, x0), 2 * static_cast<int64_t>(t0[14]));
+    round_up = c0 > a.half_drop ? 1 : 0;
+  }
+#pragma unroll
+  for (int rr = 0; rr < kRows; ++rr) {
+    const int64_t row = row_begin + rr;
+    if (row >= a.lq) break;
+    const int64_t grow = a.level + row;
+    double t[16];
+    load_record(grow, t);
+    const double q = t[14];
+    if (q == 0.0) continue;  // an integer-path row
+    const int64_t twice_q = 2 * static_cast<int64_t>(q);
+    const int64_t value = with_base(bv[rr], moddown_f64(t, xv[rr]), twice_q);
+    if (!rescale) {
+      out.at(b, row, i) = value;
+    } else if (row > 0) {
+      // ((value - c0) q0^{-1} + round_up) mod q, |value - c0| < 2^42 exact
+      double r = f64_mulmod_signed(static_cast<double>(a.rescale_inv[row - 1]),
+                                   static_cast<double>(value - c0), q, t[15]);
+      r += static_cast<double>(round_up);
+      r = r < 0.0 ? r + q : (r >= q ? r - q : r);
+      out.at(b, row - 1, i) = static_cast<int64_t>(f64_to_u64(r));
+    }
   }
   // Rows of primes above 2^46 (the base prime): the integer formula.
   for (int rr = 0; rr < kRows; ++rr) {
@@ -974,7 +1008,20 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
       xp = add_lazy(xp, mont_mul(dg[r], global(a.pdig.basis)[(r - 1) * a.pdig.basis_stride + grow], m),
                     m.twice_q);
     const int64_t xr = mont_mul(x[row * n], m.r2, m);
-    finish(rr, row, mont_mul(sub_lazy(xr, xp, m.twice_q), a.p_inverse[grow], m), m.twice_q);
+    const int64_t value = with_base(bv[rr], mont_mul(sub_lazy(xr, xp, m.twice_q), a.p_inverse[grow], m),
+                                    m.twice_q);
+    if (!rescale) {
+      out.at(b, row, i) = value;
+    } else if (row > 0) {
+      const uint64_t q = m.q;
+      int64_t d = value - c0;
+      if (d < 0) d += static_cast<int64_t>(q);
+      uint64_t r = shoup_mulmod(static_cast<uint64_t>(a.rescale_inv[row - 1]),
+                                static_cast<uint64_t>(a.rescale_inv_shoup[row - 1]), static_cast<uint64_t>(d), q);
+      r += static_cast<uint64_t>(round_up);
+      if (r >= q) r -= q;
+      out.at(b, row - 1, i) = static_cast<int64_t>(r);
+    }
   }
 }
 
@@ -1036,7 +1083,8 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
                               const int64_t* galois_index_tables, const int64_t* galois_sign_tables,
                               int64_t* workspace_digits, int64_t* workspace_inter,
                               int64_t* workspace_acc, fh_view3 base0, fh_view3 base1, fh_view3 out0,
-                              fh_view3 out1, int32_t mode, void* stream) {
+                              fh_view3 out1, int32_t mode, const int64_t* rescale_inverse,
+                              const int64_t* rescale_inverse_shoup, int64_t half_drop, void* stream) {
   FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "fused key switch supports logN 12..16");
   FH_CHECK_ARG(batch >= 1 && batch <= 32767, "fused key switch batch %lld out of range",
                (long long)batch);
@@ -1195,6 +1243,12 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   md.has_base1 = base1.data != nullptr ? 1 : 0;
   md.gal_idx = galois_index_tables;
   md.gal_sign = galois_sign_tables;
+  FH_CHECK_ARG((rescale_inverse == nullptr) == (rescale_inverse_shoup == nullptr),
+               "rescale tables go together");
+  FH_CHECK_ARG(rescale_inverse == nullptr || (lq >= 2 && half_drop > 0), "fused rescale needs two Q rows");
+  md.rescale_inv = rescale_inverse;
+  md.rescale_inv_shoup = rescale_inverse_shoup;
+  md.half_drop = half_drop;
   const int64_t md_blocks = (n / kBlock) * ceil_div(lq, kModDownRowsPerThread) * 2 * batch;
   FH_CHECK_ARG(md_blocks <= 0x7fffffff, "key switch grid too large");
   hipLaunchKernelGGL(k_ks_moddown_add, dim3(static_cast<uint32_t>(md_blocks)), dim3(kBlock), 0, s, md);

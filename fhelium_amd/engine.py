@@ -610,6 +610,12 @@ class CkksEngine:
         key = self.relinearization_key
         if key is None:
             key = self.generate_relinearization_key()
+        next_level = a.level + 1
+        if next_level >= self.config.public_level_count:
+            raise MaximumLevelError(a.level, self.config.public_level_count)
         d = self.key_switcher.tensor_product_coefficient(a.data, b.data, a.level)
-        out = self.key_switcher.key_switch_fused(d[2], key, a.level, base0=d[0], base1=d[1])
-        return self.rescale_to_next_level(Ciphertext(out, a.level, scale, "coefficient", "standard", "Q"))
+        # relinearization and rescale in one key switch (ModDown writes the
+        # rescaled rows)
+        out = self.key_switcher.key_switch_fused(d[2], key, a.level, base0=d[0], base1=d[1], rescale=True)
+        dropped = self.config.moduli[self.runtime.row_start(a.level)]
+        return Ciphertext(out, next_level, scale / float(dropped), "coefficient", "standard", "Q")

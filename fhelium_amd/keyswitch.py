@@ -98,10 +98,12 @@ class HybridKeySwitcher:
     def key_switch_fused(self, component: torch.Tensor, key: KeySwitchKey | list[KeySwitchKey],
                          level: int, base0: torch.Tensor | None = None,
                          base1: torch.Tensor | None = None,
-                         galois_elements: list[int] | None = None) -> torch.Tensor:
+                         galois_elements: list[int] | None = None, rescale: bool = False) -> torch.Tensor:
         """Whole key switch in eight launches; returns [2, *batch, Q_level, N]
         holding (base0 + corr0, base1 + corr1) canonically added, or the bare
-        corrections where a base is None.
+        corrections where a base is None.  With ``rescale`` the result is
+        also rescaled in the same ModDown launch (rescale_to_next_level's
+        nearest rounding) and has Q_level - 1 rows.
 
         With ``galois_elements`` (one per batch entry) the component and base0
         are read through entry b's coefficient automorphism, i.e. this is the
@@ -123,7 +125,10 @@ class HybridKeySwitcher:
         ws_digits = torch.empty((b, lq, n), dtype=torch.int64, device=dev)
         ws_inter = torch.empty((b, dnum, lqp, n), dtype=torch.int64, device=dev)
         ws_acc = torch.empty((2, b, lqp, n), dtype=torch.int64, device=dev)
-        out = torch.empty((2, b, lq, n), dtype=torch.int64, device=dev)
+        out_rows = lq - 1 if rescale else lq
+        if rescale and lq < 2:
+            raise ValueError("a fused rescale needs at least two Q rows")
+        out = torch.empty((2, b, out_rows, n), dtype=torch.int64, device=dev)
         if isinstance(key, (list, tuple)):
             if len(key) != b:
                 raise ValueError("one key per batch entry is required")
@@ -165,7 +170,8 @@ class HybridKeySwitcher:
                  nt.inv.data_ptr(), nt.inv_shoup.data_ptr(), key_ptr, bases_ptr,
                  ref.stride(0), ref.stride(1), ref.stride(2), gal_idx_ptr, gal_sgn_ptr,
                  ws_digits.data_ptr(), ws_inter.data_ptr(), ws_acc.data_ptr(), view(base0), view(base1),
-                 nat.v3(out[0]), nat.v3(out[1]), rt.ntt._mode, nat.stream_ptr(dev))
+                 nat.v3(out[0]), nat.v3(out[1]), rt.ntt._mode, *self._rescale_args(level, rescale),
+                 nat.stream_ptr(dev))
         if rec is not None:
             ev1.record()
             rec.append((ev0, ev1, self.algorithmic_bytes(b, lq, lqp, dnum, isinstance(key, (list, tuple)),
@@ -173,7 +179,31 @@ class HybridKeySwitcher:
         # Workspaces and the address tables may be released now: PyTorch's
         # caching allocator only hands their blocks to later work on this
         # same stream, which is ordered after these launches.
-        return out.reshape(2, *batch_shape, lq, n)
+        return out.reshape(2, *batch_shape, out_rows, n)
+
+    def _rescale_args(self, level: int, rescale: bool) -> tuple:
+        """(q_level^{-1} mod q_i table, Shoup companions, floor(q_level / 2))
+        for the rows above ``level``, cached on the device; nulls without a
+        rescale."""
+
+        if not rescale:
+            return None, None, 0
+        cache = self.__dict__.setdefault("_rescale_tables", {})
+        hit = cache.get(level)
+        if hit is None:
+            import numpy as np
+
+            cfg = self.config
+            start = self.rt.row_start(level)
+            q0 = cfg.moduli[start]
+            qs = cfg.moduli[start + 1: cfg.num_q_primes]
+            inv = [pow(q0, -1, q) for q in qs]
+            shoup = [(v << 64) // q for v, q in zip(inv, qs)]
+            inv_t = torch.tensor(inv, dtype=torch.int64).to(self.rt.device)
+            shoup_t = torch.from_numpy(np.array(shoup, dtype=np.uint64).view(np.int64)).to(self.rt.device)
+            hit = (inv_t, shoup_t, q0 // 2)
+            cache[level] = hit
+        return hit[0].data_ptr(), hit[1].data_ptr(), hit[2]
 
     def tensor_product_coefficient(self, a: torch.Tensor, b: torch.Tensor, level: int) -> torch.Tensor:
         """[3, *batch, Q_l, N] coefficient-domain d0, d1, d2 of two

@@ -196,7 +196,13 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
  *   workspace_*      digits [batch, lq, N], inter [batch, digit_count, lqp, N],
  *                    acc [2, batch, lqp, N]
  *   base0 / base1    optional canonical rows added to correction 0 / 1
- *                    (data == NULL: none); out0 / out1 receive the results. */
+ *                    (data == NULL: none); out0 / out1 receive the results.
+ *   rescale_inverse  NULL, or [lq - 1] q_level^{-1} mod q_{level+i} (standard)
+ *                    with rescale_inverse_shoup (Shoup companions) and
+ *                    half_drop = floor(q_level / 2): the results are then
+ *                    rescaled as rescale_to_next_level does
+ *                    (ckks_engine.py:1131-1174, ckks_cpu.cpp:204-275) and
+ *                    out0 / out1 receive lq - 1 rows. */
 int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp, int64_t level,
                        int64_t log_n, int64_t small_rows, const int64_t* digit_table,
                        int64_t digit_count,
@@ -208,7 +214,9 @@ int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lq
                        int64_t key_stride_row, const int64_t* galois_index_tables,
                        const int64_t* galois_sign_tables, int64_t* workspace_digits,
                        int64_t* workspace_inter, int64_t* workspace_acc, fh_view3 base0,
-                       fh_view3 base1, fh_view3 out0, fh_view3 out1, int32_t mode, void* stream);
+                       fh_view3 base1, fh_view3 out0, fh_view3 out1, int32_t mode,
+                       const int64_t* rescale_inverse, const int64_t* rescale_inverse_shoup,
+                       int64_t half_drop, void* stream);
 
 /* HMult front of the batching tier: d0 = a0 b0, d1 = a0 b1 + a1 b0,
  * d2 = a1 b1 of two coefficient-domain ciphertexts, returned in the

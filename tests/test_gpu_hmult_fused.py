@@ -54,3 +54,25 @@ def test_tensor_product_on_strided_inputs(eng):
                      eng.coefficient_domain_to_ntt_domain(type(ct)(b.contiguous(), 5, ct.scale))).data,
         eng.runtime.row_start(5))
     assert torch.equal(eng.key_switcher.tensor_product_coefficient(a, b, 5), want)
+
+
+@pytest.mark.parametrize("level", [0, 9, 31])
+def test_keyswitch_with_fused_rescale(eng, level):
+    """key_switch_fused(..., rescale=True) equals the un-rescaled key switch
+    followed by rescale_to_next_level, for a rotation-shaped call (Galois
+    gather, base0 only) and a bare call (no bases)."""
+
+    from fhelium_amd.core import Ciphertext
+
+    x = np.random.default_rng(level + 100).uniform(-1, 1, (2, eng.config.slots))
+    ct = eng.encrypt(x, level=level)
+    key = eng.rotation_key(3)
+    ks = eng.key_switcher
+    g = [key.galois_element] * 2
+    plain = ks.key_switch_fused(ct.data[1], key, level, base0=ct.data[0], galois_elements=g)
+    fused = ks.key_switch_fused(ct.data[1], key, level, base0=ct.data[0], galois_elements=g, rescale=True)
+    want = eng.rescale_to_next_level(Ciphertext(plain, level, ct.scale))
+    assert torch.equal(fused, want.data)
+    bare = ks.key_switch_fused(ct.data[1], key, level)
+    bare_r = ks.key_switch_fused(ct.data[1], key, level, rescale=True)
+    assert torch.equal(bare_r, eng.rescale_to_next_level(Ciphertext(bare, level, ct.scale)).data)
