@@ -97,6 +97,28 @@ __device__ __forceinline__ Mont load_mont(const ParamView& p, int64_t limb) {
   return m;
 }
 
+// x >= m ? x - m : x for x < 2^63 + m.  gfx950 has a one-instruction 64-bit
+// add (v_lshl_add_u64) but subtracts with a carry chain whose VCC read needs
+// wait states, and a compare-and-select costs the same wait states.  So the
+// difference is formed as x + neg_m (neg_m = 2^64 - m) and the sign word of
+// the result selects through a mask.  The empty asm statements keep the
+// compiler from folding the addition back into a subtraction and the mask
+// back into a compare-and-select; they emit no instruction.
+template <class T>
+__device__ __forceinline__ T opaque(T v) {
+  asm("" : "+v"(v));
+  return v;
+}
+
+__device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m, uint64_t neg_m) {
+  const uint64_t d = x + opaque(neg_m);
+  const uint32_t sign = opaque(static_cast<uint32_t>(static_cast<int32_t>(d >> 32) >> 31));
+  const uint64_t keep = static_cast<uint64_t>(sign) << 32 | sign;  // all ones when x < m
+  return d + (m & keep);
+}
+
+__device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m) { return csub(x, m, uint64_t{0} - m); }
+
 // REDC with R = 2^62 followed by one conditional subtraction, i.e. the
 // reference's `multiply` (montgomery.h:125-143) bit for bit, including the
 // 128-bit wrap-around of its unsigned __int128 intermediate.
@@ -112,7 +134,7 @@ __device__ __forceinline__ int64_t mont_mul(int64_t lhs, int64_t rhs,
   const uint64_t s_lo = lo + mq_lo;
   const uint64_t s_hi = hi + mq_hi + (s_lo < lo ? 1u : 0u);
   const uint64_t r = (s_lo >> kRadixBits) | (s_hi << (64 - kRadixBits));
-  return static_cast<int64_t>(r < q ? r : r - q);
+  return static_cast<int64_t>(csub(r, q));
 }
 
 __device__ __forceinline__ int64_t mont_mul(int64_t a, int64_t b,
@@ -150,19 +172,18 @@ __device__ __forceinline__ int64_t mont_mul_split(int64_t a, int64_t b,
 
 __device__ __forceinline__ int64_t add_lazy(int64_t a, int64_t b,
                                             int64_t twice_q) {
-  const int64_t s = a + b;
-  return s < twice_q ? s : s - twice_q;
+  return static_cast<int64_t>(csub(static_cast<uint64_t>(a + b), static_cast<uint64_t>(twice_q)));
 }
 
+// a + 2q - b, with 2q - b = (2q + 1) + ~b (no carry chain), then reduced.
 __device__ __forceinline__ int64_t sub_lazy(int64_t a, int64_t b,
                                             int64_t twice_q) {
-  const int64_t d = a + twice_q - b;
-  return d < twice_q ? d : d - twice_q;
+  const uint64_t d = static_cast<uint64_t>(a) + (static_cast<uint64_t>(twice_q) + 1u) + ~static_cast<uint64_t>(b);
+  return static_cast<int64_t>(csub(d, static_cast<uint64_t>(twice_q)));
 }
 
 __device__ __forceinline__ int64_t canonicalize(int64_t v, int64_t twice_q) {
-  const int64_t q = twice_q >> 1;
-  return v < q ? v : v - q;
+  return static_cast<int64_t>(csub(static_cast<uint64_t>(v), static_cast<uint64_t>(twice_q >> 1)));
 }
 
 __device__ __forceinline__ int64_t center(int64_t v, int64_t twice_q) {
@@ -212,8 +233,9 @@ __device__ __forceinline__ double f64_mulmod_signed(double w, double x, double q
 __device__ __forceinline__ uint64_t shoup_mulmod(uint64_t w, uint64_t wp,
                                                  uint64_t x, uint64_t q) {
   const uint64_t t = __umul64hi(wp, x);
-  const uint64_t r = w * x - t * q;
-  return r >= q ? r - q : r;
+  // w x - t q mod 2^64 as w x + t (2^64 - q): additions only
+  const uint64_t neg_q = opaque(uint64_t{0} - q);
+  return csub(w * x + t * neg_q, q, neg_q);
 }
 
 // Exact u64 <-> f64 conversions for integers below 2^52 (every value the

@@ -164,14 +164,9 @@ struct U64Path {
   __device__ __forceinline__ int64_t mid(V v) const { return static_cast<int64_t>(v); }
   __device__ __forceinline__ V unmid(int64_t b) const { return static_cast<uint64_t>(b); }
   __device__ __forceinline__ V mul(Tw w, V x) const { return shoup_mulmod(w.w, w.wp, x, q); }
-  __device__ __forceinline__ V add(V a, V b) const {
-    const V s = a + b;
-    return s >= twoq ? s - twoq : s;
-  }
-  __device__ __forceinline__ V sub(V a, V b) const {
-    const V d = (a + twoq) - b;
-    return d >= twoq ? d - twoq : d;
-  }
+  __device__ __forceinline__ V add(V a, V b) const { return csub(a + b, twoq); }
+  // a + 2q - b with 2q - b = (2q + 1) + ~b (no carry chain)
+  __device__ __forceinline__ V sub(V a, V b) const { return csub(a + (twoq + 1u) + ~b, twoq); }
   __device__ __forceinline__ void fwd(V& u, V& l, Tw w) const {
     const V v = mul(w, l);
     const V e = add(u, v);
@@ -187,7 +182,7 @@ struct U64Path {
     return mul(konst(row->to_mont, row->to_mont_p), x);
   }
   __device__ __forceinline__ int64_t out_fwd(V v) const {
-    if (kCanonicalOut) v = v >= q ? v - q : v;
+    if (kCanonicalOut) v = csub(v, q);
     return static_cast<int64_t>(v);
   }
   __device__ __forceinline__ int64_t out_inv(V v, int form) const {
