@@ -870,6 +870,30 @@ struct ModDownArgs {
   int64_t half_drop;
 };
 
+// Garner digits of the P residues with the digit count fixed at compile
+// time (the same recurrence as garner(); no runtime row loops).
+template <int R>
+__device__ __forceinline__ void garner_fixed(const KsDigit& d, int64_t* dg) {
+  const Mont m0 = digit_mont(d, 0);
+  int64_t first = dg[0];
+  first = first < static_cast<int64_t>(m0.q) ? first : first - static_cast<int64_t>(m0.q);
+  int64_t src[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    src[r] = dg[r];
+    dg[r] = first;
+  }
+#pragma unroll
+  for (int row = 1; row < R; ++row) {
+    const Mont mr = digit_mont(d, row);
+    const int64_t digit = mont_mul(canonical_operand(src[row] - dg[row], mr.q), global(d.normalizers)[row - 1], mr);
+    dg[row] = digit;
+#pragma unroll
+    for (int t = row + 1; t < R; ++t) dg[t] += mont_mul(digit, global(d.propagation)[(row - 1) * R + t], digit_mont(d, t));
+  }
+}
+
+template <int P>
 __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   constexpr int kRows = kModDownRowsPerThread;
   const int64_t n = int64_t{1} << a.log_n;
@@ -884,7 +908,6 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   const uint32_t cb = w / (chunks * groups);
   const int64_t c = cb / batch;
   const int64_t b = cb % batch;
-  const int p_count = static_cast<int>(a.lqp - a.lq);
   const int64_t* x = a.acc + (c * a.batch + b) * a.lqp * n + i;
   const View3& base = c == 0 ? a.base0 : a.base1;
   const bool has_base = (c == 0 ? a.has_base0 : a.has_base1) != 0;
@@ -899,9 +922,9 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     negate = global(reinterpret_cast<const int8_t*>(a.gal_sign[b]))[i] == -1;
   }
   // Issue every load of the thread first: P rows, Q rows, base rows.
-  int64_t dg[kMaxDigitRows];
+  int64_t dg[P];
 #pragma unroll
-  for (int r = 0; r < kMaxDigitRows; ++r) dg[r] = r < p_count ? x[(a.lq + r) * n] : 0;
+  for (int r = 0; r < P; ++r) dg[r] = x[(a.lq + r) * n];
   int64_t xv[kRows], bv[kRows];
 #pragma unroll
   for (int rr = 0; rr < kRows; ++rr) {
@@ -911,10 +934,10 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   }
   // Mixed-radix digits of X mod P (X_P = sum_r c_r prod_{t<r} p_t), each
   // split as c_hi 2^30 + c_lo (both exact in binary64).
-  garner(a.pdig, dg);
-  double dlo[kMaxP], dhi[kMaxP];
+  garner_fixed<P>(a.pdig, dg);
+  double dlo[P], dhi[P];
 #pragma unroll
-  for (int r = 0; r < kMaxP; ++r) {
+  for (int r = 0; r < P; ++r) {
     dlo[r] = static_cast<double>(static_cast<uint32_t>(dg[r] & ((int64_t{1} << 30) - 1)));
     dhi[r] = static_cast<double>(static_cast<uint32_t>(dg[r] >> 30));
   }
@@ -931,33 +954,22 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     if (sum >= twice_q) sum -= twice_q;
     return canonicalize(sum, twice_q);
   };
-  // floor(X / P) mod q of a binary64 row from its 16-word record: M_r,
-  // 2^30 M_r, P^{-1}, q, 1/q -- (x - X_P) P^{-1} mod q with
+  // floor(X / P) mod q of a binary64 row from its 16-word record (M_r,
+  // 2^30 M_r, P^{-1}, ..., q, 1/q): (x - X_P) P^{-1} mod q with
   // X_P mod q = sum_r c_lo M_r + c_hi 2^30 M_r, all in signed binary64
   // (|sum| < 4.2q), canonicalised once.
-  auto moddown_f64 = [&](const double (&t)[16], int64_t xval) -> int64_t {
-    const double q = t[14], qinv = t[15];
+  auto moddown_f64 = [&](Global<double>* rec, int64_t xval) -> int64_t {
+    const double q = rec[14], qinv = rec[15];
     double xp = 0.0;
-    double pinv = 0.0;
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
-      if (r == p_count) pinv = t[2 * r];
-      if (r >= p_count) continue;
-      xp += f64_mulmod_signed(t[r], dlo[r], q, qinv);
-#pragma unroll
-      for (int h = 0; h < 6; ++h)
-        if (h == p_count) xp += f64_mulmod_signed(t[h + r], dhi[r], q, qinv);
+    for (int r = 0; r < P; ++r) {
+      xp += f64_mulmod_signed(rec[r], dlo[r], q, qinv);
+      xp += f64_mulmod_signed(rec[P + r], dhi[r], q, qinv);
     }
-    if (p_count == 6) pinv = t[12];
     const double diff = u64_to_f64(static_cast<uint64_t>(xval)) - xp;
-    double res = f64_mulmod_signed(pinv, diff, q, qinv);
+    double res = f64_mulmod_signed(rec[2 * P], diff, q, qinv);
     res = res < 0.0 ? res + q : res;
     return static_cast<int64_t>(f64_to_u64(res));
-  };
-  auto load_record = [&](int64_t grow, double (&t)[16]) {
-    Global<double>* rec = global(a.pdig.moddown_f64) + grow * pitch;
-#pragma unroll
-    for (int w = 0; w < 16; ++w) t[w] = rec[w];
   };
   // Fused rescale: every thread needs the final row-0 value of its
   // coefficient (row 0 is a binary64 row: the dropped prime is a scale prime).
@@ -965,11 +977,10 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   int64_t c0 = 0;
   int64_t round_up = 0;
   if (rescale) {
-    double t0[16];
-    load_record(a.level, t0);
+    Global<double>* rec0 = global(a.pdig.moddown_f64) + a.level * pitch;
     const int64_t x0 = row_begin == 0 ? xv[0] : x[0];
     const int64_t b0 = row_begin == 0 ? bv[0] : (has_base ? base.at(b, 0, src_i) : 0);
-    c0 = with_base(b0, moddown_f64(t0, x0), 2 * static_cast<int64_t>(t0[14]));
+    c0 = with_base(b0, moddown_f64(rec0, x0), 2 * static_cast<int64_t>(static_cast<double>(rec0[14])));
     round_up = c0 > a.half_drop ? 1 : 0;
   }
 #pragma unroll
@@ -977,50 +988,47 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     const int64_t row = row_begin + rr;
     if (row >= a.lq) break;
     const int64_t grow = a.level + row;
-    double t[16];
-    load_record(grow, t);
-    const double q = t[14];
-    if (q == 0.0) continue;  // an integer-path row
-    const int64_t twice_q = 2 * static_cast<int64_t>(q);
-    const int64_t value = with_base(bv[rr], moddown_f64(t, xv[rr]), twice_q);
-    if (!rescale) {
-      out.at(b, row, i) = value;
-    } else if (row > 0) {
-      // ((value - c0) q0^{-1} + round_up) mod q, |value - c0| < 2^42 exact
-      double r = f64_mulmod_signed(static_cast<double>(a.rescale_inv[row - 1]),
-                                   static_cast<double>(value - c0), q, t[15]);
-      r += static_cast<double>(round_up);
-      r = r < 0.0 ? r + q : (r >= q ? r - q : r);
-      out.at(b, row - 1, i) = static_cast<int64_t>(f64_to_u64(r));
+    Global<double>* rec = global(a.pdig.moddown_f64) + grow * pitch;
+    const double q = rec[14];
+    int64_t value;
+    if (q != 0.0) {
+      const int64_t twice_q = 2 * static_cast<int64_t>(q);
+      value = with_base(bv[rr], moddown_f64(rec, xv[rr]), twice_q);
+      if (rescale && row > 0) {
+        // ((value - c0) q0^{-1} + round_up) mod q, |value - c0| < 2^42 exact
+        double r = f64_mulmod_signed(static_cast<double>(a.rescale_inv[row - 1]),
+                                     static_cast<double>(value - c0), q, rec[15]);
+        r += static_cast<double>(round_up);
+        r = r < 0.0 ? r + q : (r >= q ? r - q : r);
+        value = static_cast<int64_t>(f64_to_u64(r));
+      }
+    } else {
+      // A prime above 2^46 (the base prime): the integer formula.
+      const Mont m = load_mont(a.params, grow);
+      // (X_P R) mod q by the reference basis-extension formula, then
+      // ((x R) - (X_P R)) * P^{-1} with REDC = (x - X_P) P^{-1} mod q.
+      int64_t xp = mont_mul(dg[0], m.r2, m);
+#pragma unroll
+      for (int r = 1; r < P; ++r)
+        xp = add_lazy(xp, mont_mul(dg[r], global(a.pdig.basis)[(r - 1) * a.pdig.basis_stride + grow], m),
+                      m.twice_q);
+      const int64_t xr = mont_mul(xv[rr], m.r2, m);
+      value = with_base(bv[rr], mont_mul(sub_lazy(xr, xp, m.twice_q), a.p_inverse[grow], m), m.twice_q);
+      if (rescale && row > 0) {
+        const uint64_t qq = m.q;
+        int64_t d = value - c0;
+        if (d < 0) d += static_cast<int64_t>(qq);
+        uint64_t r = shoup_mulmod(static_cast<uint64_t>(a.rescale_inv[row - 1]),
+                                  static_cast<uint64_t>(a.rescale_inv_shoup[row - 1]), static_cast<uint64_t>(d), qq);
+        r += static_cast<uint64_t>(round_up);
+        if (r >= qq) r -= qq;
+        value = static_cast<int64_t>(r);
+      }
     }
-  }
-  // Rows of primes above 2^46 (the base prime): the integer formula.
-  for (int rr = 0; rr < kRows; ++rr) {
-    const int64_t row = row_begin + rr;
-    if (row >= a.lq) break;
-    const int64_t grow = a.level + row;
-    if (a.rows[grow].small) continue;
-    const Mont m = load_mont(a.params, grow);
-    // (X_P R) mod q by the reference basis-extension formula, then
-    // ((x R) - (X_P R)) * P^{-1} with REDC = (x - X_P) P^{-1} mod q.
-    int64_t xp = mont_mul(dg[0], m.r2, m);
-    for (int r = 1; r < p_count; ++r)
-      xp = add_lazy(xp, mont_mul(dg[r], global(a.pdig.basis)[(r - 1) * a.pdig.basis_stride + grow], m),
-                    m.twice_q);
-    const int64_t xr = mont_mul(x[row * n], m.r2, m);
-    const int64_t value = with_base(bv[rr], mont_mul(sub_lazy(xr, xp, m.twice_q), a.p_inverse[grow], m),
-                                    m.twice_q);
     if (!rescale) {
       out.at(b, row, i) = value;
     } else if (row > 0) {
-      const uint64_t q = m.q;
-      int64_t d = value - c0;
-      if (d < 0) d += static_cast<int64_t>(q);
-      uint64_t r = shoup_mulmod(static_cast<uint64_t>(a.rescale_inv[row - 1]),
-                                static_cast<uint64_t>(a.rescale_inv_shoup[row - 1]), static_cast<uint64_t>(d), q);
-      r += static_cast<uint64_t>(round_up);
-      if (r >= q) r -= q;
-      out.at(b, row - 1, i) = static_cast<int64_t>(r);
+      out.at(b, row - 1, i) = value;
     }
   }
 }
@@ -1251,7 +1259,15 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   md.half_drop = half_drop;
   const int64_t md_blocks = (n / kBlock) * ceil_div(lq, kModDownRowsPerThread) * 2 * batch;
   FH_CHECK_ARG(md_blocks <= 0x7fffffff, "key switch grid too large");
-  hipLaunchKernelGGL(k_ks_moddown_add, dim3(static_cast<uint32_t>(md_blocks)), dim3(kBlock), 0, s, md);
+  const dim3 md_grid(static_cast<uint32_t>(md_blocks));
+  switch (lqp - lq) {
+    case 1: hipLaunchKernelGGL(k_ks_moddown_add<1>, md_grid, dim3(kBlock), 0, s, md); break;
+    case 2: hipLaunchKernelGGL(k_ks_moddown_add<2>, md_grid, dim3(kBlock), 0, s, md); break;
+    case 3: hipLaunchKernelGGL(k_ks_moddown_add<3>, md_grid, dim3(kBlock), 0, s, md); break;
+    case 4: hipLaunchKernelGGL(k_ks_moddown_add<4>, md_grid, dim3(kBlock), 0, s, md); break;
+    case 5: hipLaunchKernelGGL(k_ks_moddown_add<5>, md_grid, dim3(kBlock), 0, s, md); break;
+    default: hipLaunchKernelGGL(k_ks_moddown_add<6>, md_grid, dim3(kBlock), 0, s, md); break;
+  }
   FH_CHECK_LAUNCH("key switch moddown + add");
   return kOk;
 }
