@@ -180,9 +180,63 @@ __device__ __forceinline__ void garner(const KsDigit& d, int64_t* dg) {
   }
 }
 
+// Garner digits with the digit count fixed at compile time (the same
+// recurrence as garner(); no runtime row loops).
+template <int R>
+__device__ __forceinline__ void garner_fixed(const KsDigit& d, int64_t* dg) {
+  const Mont m0 = digit_mont(d, 0);
+  int64_t first = dg[0];
+  first = first < static_cast<int64_t>(m0.q) ? first : first - static_cast<int64_t>(m0.q);
+  int64_t src[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    src[r] = dg[r];
+    dg[r] = first;
+  }
+#pragma unroll
+  for (int row = 1; row < R; ++row) {
+    const Mont mr = digit_mont(d, row);
+    const int64_t digit = mont_mul(canonical_operand(src[row] - dg[row], mr.q), global(d.normalizers)[row - 1], mr);
+    dg[row] = digit;
+#pragma unroll
+    for (int t = row + 1; t < R; ++t) dg[t] += mont_mul(digit, global(d.propagation)[(row - 1) * R + t], digit_mont(d, t));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 1. Garner digits.  Grid (N/256, dnum, B).
 // ---------------------------------------------------------------------------
+// Digits of one (entry, digit, coefficient) with the digit's row count R
+// fixed at compile time (R = 0: any count up to kMaxDigitRows).
+template <int R>
+__device__ __forceinline__ void decompose_one(const KsArgs& a, const KsDigit& d, int64_t b, int64_t i,
+                                              int64_t n) {
+  constexpr int kR = R == 0 ? kMaxDigitRows : R;
+  const int rows = R == 0 ? static_cast<int>(d.rows) : R;
+  int64_t dg[kR];
+#pragma unroll
+  for (int r = 0; r < kR; ++r) {
+    dg[r] = r < rows ? read_maybe_galois(a.src, a.gal_idx, a.gal_sign, b, d.row_begin + r, i,
+                                         a.params.at(kTwiceModulus, a.level + d.row_begin + r))
+                     : 0;
+  }
+  if constexpr (R == 0) {
+    garner(d, dg);
+  } else {
+    garner_fixed<R>(d, dg);
+  }
+  int64_t* out = a.digits + (b * a.lq + d.row_begin) * n + i;
+  // Digits below 2^45 are stored as binary64 (exact), the form the ModUp
+  // kernel multiplies in; the others stay integers.
+  const bool f64 = d.f64_ok != 0;
+#pragma unroll
+  for (int r = 0; r < kR; ++r) {
+    if (r < rows)
+      out[static_cast<int64_t>(r) * n] =
+          f64 ? __double_as_longlong(u64_to_f64(static_cast<uint64_t>(dg[r]))) : dg[r];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_ks_decompose(KsArgs a) {
   const int64_t n = int64_t{1} << a.log_n;
   // One XCD walks all coefficient chunks of an (entry, digit): a Galois
@@ -193,24 +247,13 @@ __global__ __launch_bounds__(kBlock) void k_ks_decompose(KsArgs a) {
   const int64_t i = static_cast<int64_t>(w % chunks) * kBlock + threadIdx.x;
   const KsDigit& d = a.dig[(w / chunks) % dnum];
   const int64_t b = w / (chunks * dnum);
-  const int rows = static_cast<int>(d.rows);
-  int64_t dg[kMaxDigitRows];
-#pragma unroll
-  for (int r = 0; r < kMaxDigitRows; ++r) {
-    dg[r] = r < rows ? read_maybe_galois(a.src, a.gal_idx, a.gal_sign, b, d.row_begin + r, i,
-                                         a.params.at(kTwiceModulus, a.level + d.row_begin + r))
-                     : 0;
-  }
-  garner(d, dg);
-  int64_t* out = a.digits + (b * a.lq + d.row_begin) * n + i;
-  // Digits below 2^45 are stored as binary64 (exact), the form the ModUp
-  // kernel multiplies in; the others stay integers.
-  const bool f64 = d.f64_ok != 0;
-#pragma unroll
-  for (int r = 0; r < kMaxDigitRows; ++r) {
-    if (r < rows)
-      out[static_cast<int64_t>(r) * n] =
-          f64 ? __double_as_longlong(u64_to_f64(static_cast<uint64_t>(dg[r]))) : dg[r];
+  // the row count is uniform over the block: branch once to a fixed-size body
+  switch (static_cast<int>(d.rows)) {
+    case 1: decompose_one<1>(a, d, b, i, n); break;
+    case 2: decompose_one<2>(a, d, b, i, n); break;
+    case 3: decompose_one<3>(a, d, b, i, n); break;
+    case 4: decompose_one<4>(a, d, b, i, n); break;
+    default: decompose_one<0>(a, d, b, i, n); break;
   }
 }
 
@@ -241,7 +284,7 @@ __device__ __forceinline__ uint64_t extend_integer(const KsDigit& d, const NttRo
 // 2. Basis extension + forward NTT column pass.
 //    Grid (T column groups, Lqp destination rows, dnum * B).
 // ---------------------------------------------------------------------------
-template <int S1, class P>
+template <int S1, int kRows, class P>
 __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const KsDigit& d,
                                               int64_t di, int64_t b, int64_t j, int64_t* lds) {
   constexpr int R1 = 1 << S1;
@@ -254,7 +297,9 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
   const int64_t* tw = a.fwd_w + grow * n;
   const int64_t* twp = a.fwd_wp + grow * n;
   const int64_t* dig0 = a.digits + (b * a.lq + d.row_begin) * n;
-  const int rows = static_cast<int>(d.rows);
+  // kRows: the digit's row count when fixed at compile time (0: runtime)
+  const int rows = kRows == 0 ? static_cast<int>(d.rows) : kRows;
+  constexpr int kMaxR = kRows == 0 ? kMaxDigitRows : kRows;
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int col = blockIdx.x * C + c;
@@ -273,7 +318,7 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
         v[k] = std::is_same<P, F64Fast>::value ? f64_mulmod_signed(c0, x, q, qinv) : f64_mulmod(c0, x, q, qinv);
       }
 #pragma unroll
-      for (int r = 1; r < kMaxDigitRows; ++r) {
+      for (int r = 1; r < kMaxR; ++r) {
         if (r >= rows) break;
         const double cr = static_cast<double>(global(d.basis_std)[(r - 1) * d.basis_stride + grow]);
         const int64_t* src = dig0 + r * n;
@@ -340,14 +385,19 @@ __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j
   const int64_t di = blockIdx.z % dnum;
   const KsDigit& d = a.dig[di];
   const NttRow& row = a.rows[a.level + j];
-  if constexpr (kSmall) {
-    if constexpr (MODE == 1) {
-      modup_columns<S1>(F64Fast(row), a, d, di, b, j, lds);
-    } else {
-      modup_columns<S1>(F64Exact(row), a, d, di, b, j, lds);
+  if constexpr (kSmall && MODE == 1) {
+    // the digit width is uniform over the block: a fixed-width body lets
+    // every digit load issue before the extension arithmetic
+    switch (static_cast<int>(d.rows)) {
+      case 1: modup_columns<S1, 1>(F64Fast(row), a, d, di, b, j, lds); break;
+      case 2: modup_columns<S1, 2>(F64Fast(row), a, d, di, b, j, lds); break;
+      case 4: modup_columns<S1, 4>(F64Fast(row), a, d, di, b, j, lds); break;
+      default: modup_columns<S1, 0>(F64Fast(row), a, d, di, b, j, lds); break;
     }
+  } else if constexpr (kSmall) {
+    modup_columns<S1, 0>(F64Exact(row), a, d, di, b, j, lds);
   } else {
-    modup_columns<S1>(U64Path<false>(row), a, d, di, b, j, lds);
+    modup_columns<S1, 0>(U64Path<false>(row), a, d, di, b, j, lds);
   }
 }
 
@@ -869,29 +919,6 @@ struct ModDownArgs {
   const int64_t* rescale_inv_shoup;  // [lq - 1] Shoup companions
   int64_t half_drop;
 };
-
-// Garner digits of the P residues with the digit count fixed at compile
-// time (the same recurrence as garner(); no runtime row loops).
-template <int R>
-__device__ __forceinline__ void garner_fixed(const KsDigit& d, int64_t* dg) {
-  const Mont m0 = digit_mont(d, 0);
-  int64_t first = dg[0];
-  first = first < static_cast<int64_t>(m0.q) ? first : first - static_cast<int64_t>(m0.q);
-  int64_t src[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    src[r] = dg[r];
-    dg[r] = first;
-  }
-#pragma unroll
-  for (int row = 1; row < R; ++row) {
-    const Mont mr = digit_mont(d, row);
-    const int64_t digit = mont_mul(canonical_operand(src[row] - dg[row], mr.q), global(d.normalizers)[row - 1], mr);
-    dg[row] = digit;
-#pragma unroll
-    for (int t = row + 1; t < R; ++t) dg[t] += mont_mul(digit, global(d.propagation)[(row - 1) * R + t], digit_mont(d, t));
-  }
-}
 
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
