@@ -2,7 +2,8 @@
 the reference engine methods, and decrypted error against an FP64 cleartext.
 
 Parity (logN=12 over the full L34 chain, 39 QP rows, identical keys and
-inputs): multiply, relinearize, rescale_to_next_level, rotate_by_step and
+inputs; and at logN=16 for the fused HMult and rotation at levels 0 and
+30): multiply, relinearize, rescale_to_next_level, rotate_by_step and
 hoisted rotations reproduce the reference CPU residues exactly; the NTT
 conversion is exact in exact mode and equal modulo q (canonical) in fast
 mode.  Error (logN=16, the L34 preset): decrypted max-abs error of
@@ -200,3 +201,30 @@ def test_fused_keyswitch_matches_reference_sequence(mode, level):
         m0, m1 = eng.key_switcher.apply_key_switch(comp, keys, level)
         np.testing.assert_array_equal(_np(fused_multi[0]), _np(m0))
         np.testing.assert_array_equal(_np(fused_multi[1]), _np(m1))
+
+
+@pytest.fixture(scope="module")
+def full_parity():
+    eng = CkksEngine(H.config(16), seed=17)
+    eng.generate_secret_key()
+    eng.generate_relinearization_key()
+    return eng, EngineRef(H.host_tables(16))
+
+
+@pytest.mark.parametrize("level", [0, 30])
+def test_full_size_fused_hmult_and_rotation_bit_exact(full_parity, level):
+    """logN=16, L34: the fused HMult (tensor product front, relinearization
+    and rescale in the key switch) and a fused rotation reproduce the
+    oracle's residues of the reference composition bit for bit."""
+
+    eng, ref = full_parity
+    a, _ = _fresh(eng, level, batch=(1,), seed=3)
+    b, _ = _fresh(eng, level, batch=(1,), seed=4)
+    got = eng.hmult(a, b)
+    rprod = ref.multiply(ref.coefficient_domain_to_ntt_domain(_np(a.data), level),
+                         ref.coefficient_domain_to_ntt_domain(_np(b.data), level), level)
+    want = ref.rescale(ref.relinearize(rprod, _np(eng.relinearization_key.data), level), level)
+    np.testing.assert_array_equal(_np(got.data), want)
+    key = eng.rotation_key(5)
+    rot = eng.rotate_by_step(got, 5)
+    np.testing.assert_array_equal(_np(rot.data), ref.rotate(want, _np(key.data), key.galois_element, level + 1))
