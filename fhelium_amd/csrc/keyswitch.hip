@@ -1322,14 +1322,27 @@ FH_API int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, 
   const int64_t n = int64_t{1} << log_n;
   const int64_t comp = batch * limbs * n;
   const NttRow* rows = static_cast<const NttRow*>(ntt_rows);
-  const int64_t* srcs[4] = {a, a + a_comp_stride, b, b + b_comp_stride};
-  const int64_t sbs[4] = {a_batch_stride, a_batch_stride, b_batch_stride, b_batch_stride};
-  const int64_t sls[4] = {a_limb_stride, a_limb_stride, b_limb_stride, b_limb_stride};
-  for (int c = 0; c < 4; ++c) {
-    int st = ntt::forward_columns_internal(workspace + c * comp, srcs[c], batch, limbs, static_cast<int>(log_n),
-                                           limbs * n, n, sbs[c], sls[c], rows, forward_twiddles,
-                                           forward_twiddles_shoup, row_start, true, s);
-    if (st != kOk) return st;
+  // Column passes of a0, a1 (workspace components 0, 1) and b0, b1 (2, 3):
+  // one launch per operand when its two components are one [2B] batch.
+  const int64_t* ops[2] = {a, b};
+  const int64_t cs[2] = {a_comp_stride, b_comp_stride};
+  const int64_t bs[2] = {a_batch_stride, b_batch_stride};
+  const int64_t ls[2] = {a_limb_stride, b_limb_stride};
+  for (int o = 0; o < 2; ++o) {
+    int64_t* dst = workspace + 2 * o * comp;
+    if (cs[o] == batch * bs[o]) {
+      int st = ntt::forward_columns_internal(dst, ops[o], 2 * batch, limbs, static_cast<int>(log_n), limbs * n, n,
+                                             bs[o], ls[o], rows, forward_twiddles, forward_twiddles_shoup,
+                                             row_start, true, s);
+      if (st != kOk) return st;
+      continue;
+    }
+    for (int c = 0; c < 2; ++c) {
+      int st = ntt::forward_columns_internal(dst + c * comp, ops[o] + c * cs[o], batch, limbs,
+                                             static_cast<int>(log_n), limbs * n, n, bs[o], ls[o], rows,
+                                             forward_twiddles, forward_twiddles_shoup, row_start, true, s);
+      if (st != kOk) return st;
+    }
   }
   HmArgs h;
   h.ws = workspace;
