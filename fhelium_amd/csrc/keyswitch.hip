@@ -100,7 +100,29 @@ struct KsArgs {
   int64_t key_sd, key_sc, key_sr;   // key strides (digit, component, row)
   const int64_t* gal_idx;   // per-entry int32 gather table addresses [B] or null
   const int64_t* gal_sign;  // per-entry int8 sign table addresses [B]
+  int64_t small_rows;       // leading QP rows on the binary64 path
 };
+
+// With fast arithmetic the binary64 rows of the ModUp output travel packed:
+// canonical residues (< 2^41) as a u32 plane and a u16 plane, 6 bytes per
+// coefficient instead of 8.  In the [Lqp, N] int64 slab of one (entry,
+// digit) the u32 plane starts at the slab and the u16 plane follows it; both
+// fit inside the slab's first small_rows rows, so the integer rows keep
+// their int64 layout after them.  Within a row the coefficients are stored
+// in 8 x 16 blocks of the 2^(logN-8) x 256 matrix (row i, column c): a band
+// of 8 rows is the 8 tiles one MAC block reads, and each 16-column block
+// row is written by one ModUp wave, so both kernels move whole cache lines.
+struct PackedInter {
+  uint32_t* lo;
+  uint16_t* hi;
+};
+__device__ __forceinline__ PackedInter packed_slab(int64_t* slab, int64_t small_rows, int64_t n) {
+  uint32_t* lo = reinterpret_cast<uint32_t*>(slab);
+  return PackedInter{lo, reinterpret_cast<uint16_t*>(lo + small_rows * n)};
+}
+__device__ __forceinline__ int64_t packed_index(int64_t i, int64_t c) {
+  return ((i >> 3) << 11) | ((c >> 4) << 7) | ((i & 7) << 4) | (c & 15);
+}
 
 // Block i runs on XCD i % 8; give each XCD one contiguous range of work
 // items (batch fastest) so consecutive items on an XCD share key tiles in
@@ -371,8 +393,20 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
       p.fwd(v[k], v[k + half], p.tw(tw, twp, (1 << s) + ((16 * t + k) >> (S1 - s))));
     }
   }
+  if constexpr (std::is_same<P, F64Fast>::value) {
+    const PackedInter pk = packed_slab(a.inter + (b * a.dnum + di) * a.lqp * n, a.small_rows, n);
+    const int64_t row0 = j * n;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) out[static_cast<int64_t>(16 * t + k) * kTile + col] = p.mid(v[k]);
+    for (int k = 0; k < 16; ++k) {
+      const uint64_t u = f64_to_u64(p.canon(v[k]));
+      const int64_t idx = row0 + packed_index(16 * t + k, col);
+      pk.lo[idx] = static_cast<uint32_t>(u);
+      pk.hi[idx] = static_cast<uint16_t>(u >> 32);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) out[static_cast<int64_t>(16 * t + k) * kTile + col] = p.mid(v[k]);
+  }
 }
 
 // One launch per arithmetic path, as for the tile pass below.
@@ -484,8 +518,30 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     Global<int64_t>* k1 = k0 + a.key_sc;
     V v[8];
     int64_t w0[8], w1[8];
+    if constexpr (std::is_same<P, F64Fast>::value) {
+      // Eight consecutive coefficients 8t..8t+7 of the tile are contiguous
+      // in the packed block layout (one 32-byte and one 16-byte load); an
+      // LDS pass moves them to the round-1 ownership.
+      const PackedInter pk = packed_slab(a.inter + (b * a.dnum + di) * a.lqp * n, a.small_rows, n);
+      const int64_t idx = j * n + packed_index(r, 8 * t);
+      const uint4 l0 = *reinterpret_cast<const uint4*>(pk.lo + idx);
+      const uint4 l1 = *reinterpret_cast<const uint4*>(pk.lo + idx + 4);
+      const uint4 hh = *reinterpret_cast<const uint4*>(pk.hi + idx);
+      const uint32_t lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+      const uint32_t hi2[4] = {hh.x, hh.y, hh.z, hh.w};
+      __syncthreads();  // the previous digit's last reads of the exchange buffer are done
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(tile[mac_e1(t, k)]);
+      for (int k = 0; k < 8; ++k) {
+        const uint64_t h = (hi2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+        sh[tile_pad(mac_e3(t, k))] = __double_as_longlong(u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32)));
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = p.unmid(tile[mac_e1(t, k)]);
+    }
     // Binary64 rows issue the key loads with the tile loads; the integer
     // rows (whose products need many more registers) load them at the MAC.
     if constexpr (kF64) {
@@ -1162,6 +1218,7 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   a.key_sr = key_stride_row;
   a.gal_idx = galois_index_tables;
   a.gal_sign = galois_sign_tables;
+  a.small_rows = small_rows;
 
   FH_CHECK_ARG(small_rows >= 0 && small_rows <= lqp, "bad binary64 row count");
   const int s1 = static_cast<int>(log_n) - kTileLog;
