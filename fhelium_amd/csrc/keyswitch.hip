@@ -394,15 +394,37 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
     }
   }
   if constexpr (std::is_same<P, F64Fast>::value) {
-    const PackedInter pk = packed_slab(a.inter + (b * a.dnum + di) * a.lqp * n, a.small_rows, n);
-    const int64_t row0 = j * n;
+    // Through LDS to run ownership: the block's R1 x C values form
+    // R1 * C / 16 = 256 runs of 16 columns of one row, each 16 consecutive
+    // packed coefficients (64 + 32 bytes); thread u writes run u.
+    constexpr int kPitch = C + 1;
+    constexpr int kRunsPerRow = C / 16;
+    static_assert(R1 * kRunsPerRow == kBlock, "one packed run per thread");
+    __syncthreads();  // every round-B read of the exchange buffer is done
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const uint64_t u = f64_to_u64(p.canon(v[k]));
-      const int64_t idx = row0 + packed_index(16 * t + k, col);
-      pk.lo[idx] = static_cast<uint32_t>(u);
-      pk.hi[idx] = static_cast<uint16_t>(u >> 32);
+    for (int k = 0; k < 16; ++k) lds[(16 * t + k) * kPitch + c] = static_cast<int64_t>(f64_to_u64(p.canon(v[k])));
+    __syncthreads();
+    const PackedInter pk = packed_slab(a.inter + (b * a.dnum + di) * a.lqp * n, a.small_rows, n);
+    const int i = threadIdx.x / kRunsPerRow;
+    const int c16 = (threadIdx.x % kRunsPerRow) * 16;
+    uint32_t lo[16], hi[8];
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) {
+      const uint64_t u = static_cast<uint64_t>(lds[i * kPitch + c16 + cc]);
+      lo[cc] = static_cast<uint32_t>(u);
+      if (cc & 1) {
+        hi[cc >> 1] |= static_cast<uint32_t>(u >> 32) << 16;
+      } else {
+        hi[cc >> 1] = static_cast<uint32_t>(u >> 32) & 0xffffu;
+      }
     }
+    const int64_t idx = j * n + packed_index(i, static_cast<int64_t>(blockIdx.x) * C + c16);
+    uint4* dlo = reinterpret_cast<uint4*>(pk.lo + idx);
+    uint4* dhi = reinterpret_cast<uint4*>(pk.hi + idx);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dlo[q] = make_uint4(lo[4 * q], lo[4 * q + 1], lo[4 * q + 2], lo[4 * q + 3]);
+    dhi[0] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    dhi[1] = make_uint4(hi[4], hi[5], hi[6], hi[7]);
   } else {
 #pragma unroll
     for (int k = 0; k < 16; ++k) out[static_cast<int64_t>(16 * t + k) * kTile + col] = p.mid(v[k]);
@@ -412,7 +434,8 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
 // One launch per arithmetic path, as for the tile pass below.
 template <int S1, int MODE, bool kSmall>
 __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j_begin) {
-  __shared__ int64_t lds[(1 << S1) * (kBlock / ((1 << S1) / 16))];
+  // (+1 column of padding: the packed store's transpose pitch)
+  __shared__ int64_t lds[(1 << S1) * (kBlock / ((1 << S1) / 16) + 1)];
   const int64_t j = j_begin + blockIdx.y;
   const uint32_t dnum = static_cast<uint32_t>(a.dnum);
   const int64_t b = blockIdx.z / dnum;
