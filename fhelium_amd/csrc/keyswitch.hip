@@ -454,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j
   } else if constexpr (kSmall) {
     modup_columns<S1, 0>(F64Exact(row), a, d, di, b, j, lds);
   } else {
-    modup_columns<S1, 0>(U64Path<false>(row), a, d, di, b, j, lds);
+    modup_columns<S1, 0>(U64Fast(row), a, d, di, b, j, lds);
   }
 }
 
@@ -623,7 +623,8 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
         mac_f64(facc0[k], v[k], w0[k], row.qd, row.qinv);
         mac_f64(facc1[k], v[k], w1[k], row.qd, row.qinv);
       } else {
-        const int64_t yi = static_cast<int64_t>(v[k]);
+        // forward outputs lie in [0, 4q): halve to [0, 2q) for the REDC bound
+        const int64_t yi = static_cast<int64_t>(p.half(v[k]));
         w0[k] = k0[mac_e3(t, k)];
         w1[k] = k1[mac_e3(t, k)];
         uacc0[k] = static_cast<uint64_t>(
@@ -707,7 +708,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     inverse_tile(pi, facc0, out0);
     inverse_tile(pi, facc1, out1);
   } else {
-    const U64Path<false> pi(row);
+    const U64Fast pi(row);
     inverse_tile(pi, uacc0, out0);
     inverse_tile(pi, uacc1, out1);
   }
@@ -741,7 +742,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSmall ?
       tiles_mac(F64Exact(row), a, g, j, b, lds);
     }
   } else {
-    tiles_mac(U64Path<false>(row), a, g, j, b, lds);
+    tiles_mac(U64Fast(row), a, g, j, b, lds);
   }
 }
 
@@ -863,9 +864,13 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
         d1[k] = f64_mulmod_signed(x1[k], yb, row.qd, row.qinv);
       }
     } else {
+      // forward outputs lie in [0, 4q): halve to [0, 2q) for the REDC bound
       const Mont mont = load_mont(a.params, grow);
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
+        x0[k] = p.half(x0[k]);
+        x1[k] = p.half(x1[k]);
+        y[k] = p.half(y[k]);
         d0[k] = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(x0[k]), static_cast<int64_t>(y[k]), mont));
         d1[k] = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(x1[k]), static_cast<int64_t>(y[k]), mont));
       }
@@ -882,6 +887,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
       const Mont mont = load_mont(a.params, grow);
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
+        y[k] = p.half(y[k]);
         d1[k] = static_cast<uint64_t>(add_lazy(
             static_cast<int64_t>(d1[k]), mont_mul(static_cast<int64_t>(x0[k]), static_cast<int64_t>(y[k]), mont),
             mont.twice_q));
@@ -969,7 +975,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSmall ?
   if constexpr (kSmall) {
     tiles_hmult(F64Fast(row), a, g, j, b, lds);
   } else {
-    tiles_hmult(U64Path<false>(row), a, g, j, b, lds);
+    tiles_hmult(U64Fast(row), a, g, j, b, lds);
   }
 }
 

@@ -17,8 +17,9 @@
 //    butterflies reduce U + L by one rounding step.  Outputs are canonical
 //    [0, q): the inverse outputs equal the reference's exactly (they are
 //    canonical there too), forward outputs equal the reference's lazy
-//    values modulo q.  Larger primes always use the exact integer path and
-//    canonicalise the forward output in this mode.
+//    values modulo q.  Larger primes use Shoup products with Harvey's lazy
+//    butterflies (forward values below 4q, inverse values below 2q) and
+//    canonical outputs in this mode.
 //
 // Execution (N = 2^L, 12 <= L <= 16): two fused passes.
 //   * "column" pass: the first L-8 forward stages (last L-8 inverse stages)
@@ -307,10 +308,10 @@ enum Mode : int { kExact = 0, kFast = 1 };
       }                                                    \
     } else {                                               \
       if (MODE == kFast) {                                 \
-        U64Path<true> p(ROW);                              \
+        U64Fast p(ROW);                                    \
         CALL;                                              \
       } else {                                             \
-        U64Path<false> p(ROW);                             \
+        U64Path p(ROW);                                    \
         CALL;                                              \
       }                                                    \
     }                                                      \
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(NttArgs a, int stage, int forw
   if (rr.row->small) {
     stage_body(F64Exact(*rr.row), rr.base, rr.tw, rr.twp, a.log_n, stage, forward != 0, b);
   } else {
-    stage_body(U64Path<false>(*rr.row), rr.base, rr.tw, rr.twp, a.log_n, stage, forward != 0, b);
+    stage_body(U64Path(*rr.row), rr.base, rr.tw, rr.twp, a.log_n, stage, forward != 0, b);
   }
 }
 
@@ -411,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_scale(NttArgs a, int what) {
     F64Exact p(row);
     *x = what == -1 ? p.out_fwd(p.to_mont(p.in(*x))) : p.out_inv(p.in(*x), what);
   } else {
-    U64Path<false> p(row);
+    U64Path p(row);
     *x = what == -1 ? p.out_fwd(p.to_mont(p.in(*x))) : p.out_inv(p.in(*x), what);
   }
 }

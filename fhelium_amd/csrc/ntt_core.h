@@ -142,9 +142,7 @@ struct F64Fast {
   }
 };
 
-// Integer Shoup path (any q < 2^62), reference lazy semantics.  With
-// kCanonicalOut the forward output is reduced to [0, q).
-template <bool kCanonicalOut>
+// Integer Shoup path (any q < 2^62), reference lazy semantics (exact mode).
 struct U64Path {
   using V = uint64_t;
   struct Tw {
@@ -181,10 +179,62 @@ struct U64Path {
   __device__ __forceinline__ V to_mont(V x) const {
     return mul(konst(row->to_mont, row->to_mont_p), x);
   }
-  __device__ __forceinline__ int64_t out_fwd(V v) const {
-    if (kCanonicalOut) v = csub(v, q);
-    return static_cast<int64_t>(v);
+  __device__ __forceinline__ int64_t out_fwd(V v) const { return static_cast<int64_t>(v); }
+  __device__ __forceinline__ int64_t out_inv(V v, int form) const {
+    const Tw c = form == kOutMontgomery ? konst(row->ninv_mont, row->ninv_mont_p)
+                                        : konst(row->ninv_std, row->ninv_std_p);
+    int64_t r = static_cast<int64_t>(mul(c, v));
+    return form == kOutCentered ? center_canonical(r, q) : r;
   }
+};
+
+// Integer Shoup path for fast arithmetic (q < 2^60), with Harvey's lazy
+// butterflies: forward values stay in [0, 4q), inverse values in [0, 2q),
+// and the Shoup product w x - hi64(w' x) q is left in [0, 2q) (any x below
+// 2^64, w < q).  One conditional subtraction per butterfly instead of three;
+// forward and inverse outputs are canonical, so they equal the reference's
+// values modulo q exactly as U64Path's do.
+struct U64Fast {
+  using V = uint64_t;
+  struct Tw {
+    uint64_t w, wp;
+  };
+  uint64_t q, twoq;
+  const NttRow* row;
+  __device__ __forceinline__ explicit U64Fast(const NttRow& r) : q(r.q), twoq(r.twice_q), row(&r) {}
+  __device__ __forceinline__ Tw tw(const int64_t* w, const int64_t* wp, int64_t idx) const {
+    return Tw{static_cast<uint64_t>(w[idx]), static_cast<uint64_t>(wp[idx])};
+  }
+  __device__ __forceinline__ Tw tw_make(int64_t w, int64_t wp) const {
+    return Tw{static_cast<uint64_t>(w), static_cast<uint64_t>(wp)};
+  }
+  __device__ __forceinline__ Tw konst(uint64_t c, uint64_t cp) const { return Tw{c, cp}; }
+  __device__ __forceinline__ V in(int64_t raw) const { return static_cast<uint64_t>(raw); }
+  __device__ __forceinline__ int64_t mid(V v) const { return static_cast<int64_t>(v); }
+  __device__ __forceinline__ V unmid(int64_t b) const { return static_cast<uint64_t>(b); }
+  // w x - t q with t = hi64(w' x), formed as w x + t (2^64 - q): in [0, 2q)
+  __device__ __forceinline__ V mul_lazy(Tw w, V x) const {
+    const uint64_t t = __umul64hi(w.wp, x);
+    return w.w * x + t * opaque(uint64_t{0} - q);
+  }
+  __device__ __forceinline__ V mul(Tw w, V x) const { return csub(mul_lazy(w, x), q); }
+  // [0, 4q) -> [0, 2q)
+  __device__ __forceinline__ V half(V v) const { return csub(v, twoq); }
+  __device__ __forceinline__ void fwd(V& u, V& l, Tw w) const {
+    const V a = half(u);
+    const V v = mul_lazy(w, l);
+    u = a + v;
+    l = a + (twoq + 1u) + ~v;  // a - v + 2q without a borrow chain
+  }
+  __device__ __forceinline__ void inv(V& u, V& l, Tw w) const {
+    const V d = u + (twoq + 1u) + ~l;
+    u = half(u + l);
+    l = mul_lazy(w, d);
+  }
+  __device__ __forceinline__ V to_mont(V x) const {
+    return mul_lazy(konst(row->to_mont, row->to_mont_p), x);
+  }
+  __device__ __forceinline__ int64_t out_fwd(V v) const { return static_cast<int64_t>(csub(half(v), q)); }
   __device__ __forceinline__ int64_t out_inv(V v, int form) const {
     const Tw c = form == kOutMontgomery ? konst(row->ninv_mont, row->ninv_mont_p)
                                         : konst(row->ninv_std, row->ninv_std_p);
