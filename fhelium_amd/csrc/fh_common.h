@@ -238,6 +238,36 @@ __device__ __forceinline__ uint64_t shoup_mulmod(uint64_t w, uint64_t wp,
   return csub(w * x + t * neg_q, q, neg_q);
 }
 
+// Exact sum of 64 x 64-bit products, for one REDC after many products.  The
+// sum is held as c2 2^64 + c1 2^32 + c0 with 32-bit limbs c0, c1; each step
+// is four 32 x 32 + 64-bit multiply-adds (a b + c + d < 2^64 for 32-bit a, b,
+// c, d), so no carry flag is ever produced or read.  c2 only grows: the
+// caller bounds the total below 2^128.
+struct WideAcc {
+  uint32_t c0, c1;
+  uint64_t c2;
+};
+
+__device__ __forceinline__ void wide_mac(WideAcc& a, uint64_t y, uint64_t w) {
+  const uint32_t y0 = static_cast<uint32_t>(y), y1 = static_cast<uint32_t>(y >> 32);
+  const uint32_t w0 = static_cast<uint32_t>(w), w1 = static_cast<uint32_t>(w >> 32);
+  const uint64_t t = static_cast<uint64_t>(y0) * w0 + a.c0;
+  const uint64_t u = static_cast<uint64_t>(y0) * w1 + (static_cast<uint64_t>(a.c1) + (t >> 32));
+  const uint64_t v = static_cast<uint64_t>(y1) * w0 + static_cast<uint32_t>(u);
+  a.c0 = static_cast<uint32_t>(t);
+  a.c1 = static_cast<uint32_t>(v);
+  a.c2 += static_cast<uint64_t>(y1) * w1 + (u >> 32) + (v >> 32);
+}
+
+// (sum + m q) / 2^64 with m = -sum q^{-1} mod 2^64: congruent to sum 2^-64,
+// below sum / 2^64 + q.  The low word of sum + m q is zero, so its carry is
+// (low word of sum != 0).
+__device__ __forceinline__ uint64_t wide_redc64(const WideAcc& a, uint64_t q, uint64_t neg_qinv64) {
+  const uint64_t lo = static_cast<uint64_t>(a.c1) << 32 | a.c0;
+  const uint64_t m = lo * neg_qinv64;
+  return a.c2 + __umul64hi(m, q) + (lo != 0 ? 1u : 0u);
+}
+
 // Exact u64 <-> f64 conversions for integers below 2^52 (every value the
 // binary64 paths convert is below 2^47): 2^52 + v has the bit pattern
 // 0x4330000000000000 | v, so each direction is one integer and one binary64
@@ -269,7 +299,8 @@ struct NttRow {
   uint64_t ninv_std_p;
   uint64_t ninv_std_rinv;    // N^{-1} R^{-2} mod q (standard output of a
   uint64_t ninv_std_rinv_p;  // binary64 key MAC that skipped REDC's R^{-1})
-  uint64_t reserved[3];
+  uint64_t neg_qinv64;       // -q^{-1} mod 2^64 (REDC with radix 2^64)
+  uint64_t reserved[2];
 };
 static_assert(sizeof(NttRow) == 128, "NttRow must stay 128 bytes");
 

@@ -363,10 +363,34 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
       }
     }
   } else {
+    // Integer rows (fast path): each term d_r (M_r R mod q) is a Shoup
+    // product left in [0, 2q); the sum (below 2 rows q <= 16q < 2^64) is
+    // reduced once into [0, 4q), the forward butterflies' input range.
+    static_assert(std::is_same<P, U64Fast>::value, "integer rows take the lazy fast path");
+    const uint64_t q = row.q;
+    const uint64_t neg_q = opaque(uint64_t{0} - q);
+    auto term = [&](uint64_t w, uint64_t wp, uint64_t x) { return w * x + __umul64hi(wp, x) * neg_q; };
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int64_t pos = static_cast<int64_t>(t + T * k) * kTile + col;
-      v[k] = extend_integer(d, row, grow, dig0, n, pos);
+      v[k] = term(row.to_mont, row.to_mont_p, digit_u64(d, dig0[pos]));
+    }
+#pragma unroll
+    for (int r = 1; r < kMaxR; ++r) {
+      if (r >= rows) break;
+      const int64_t cidx = (r - 1) * d.basis_stride + grow;
+      const uint64_t cr = static_cast<uint64_t>(global(d.basis_std)[cidx]);
+      const uint64_t crp = static_cast<uint64_t>(global(d.basis_shoup)[cidx]);
+      const int64_t* src = dig0 + r * n;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int64_t pos = static_cast<int64_t>(t + T * k) * kTile + col;
+        v[k] += term(cr, crp, digit_u64(d, src[pos]));
+      }
+    }
+    if (rows > 2) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = csub(csub(v[k], 8 * q), 4 * q);
     }
   }
 #pragma unroll
@@ -454,7 +478,11 @@ __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j
   } else if constexpr (kSmall) {
     modup_columns<S1, 0>(F64Exact(row), a, d, di, b, j, lds);
   } else {
-    modup_columns<S1, 0>(U64Fast(row), a, d, di, b, j, lds);
+    switch (static_cast<int>(d.rows)) {
+      case 1: modup_columns<S1, 1>(U64Fast(row), a, d, di, b, j, lds); break;
+      case 2: modup_columns<S1, 2>(U64Fast(row), a, d, di, b, j, lds); break;
+      default: modup_columns<S1, 0>(U64Fast(row), a, d, di, b, j, lds); break;
+    }
   }
 }
 
@@ -525,13 +553,13 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
   };
   Global<int64_t>* key = global(a.key_bases ? reinterpret_cast<const int64_t*>(a.key_bases[b]) : a.key);
   const int64_t tile_off = static_cast<int64_t>(r) * kTile;
-  const Mont mont = load_mont(a.params, grow);
   double facc0[8], facc1[8];
   uint64_t uacc0[8], uacc1[8];
+  WideAcc wacc0[8], wacc1[8];  // integer rows: exact sums, one REDC after the digits
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     facc0[k] = facc1[k] = 0.0;
-    uacc0[k] = uacc1[k] = 0;
+    wacc0[k] = wacc1[k] = WideAcc{0, 0, 0};
   }
   const int64_t* tile = a.inter + ((b * a.dnum) * a.lqp + j) * n + tile_off;
   const int64_t digit_stride = a.lqp * n;
@@ -623,15 +651,32 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
         mac_f64(facc0[k], v[k], w0[k], row.qd, row.qinv);
         mac_f64(facc1[k], v[k], w1[k], row.qd, row.qinv);
       } else {
-        // forward outputs lie in [0, 4q): halve to [0, 2q) for the REDC bound
-        const int64_t yi = static_cast<int64_t>(p.half(v[k]));
         w0[k] = k0[mac_e3(t, k)];
         w1[k] = k1[mac_e3(t, k)];
-        uacc0[k] = static_cast<uint64_t>(
-            add_lazy(static_cast<int64_t>(uacc0[k]), mont_mul(yi, w0[k], mont), mont.twice_q));
-        uacc1[k] = static_cast<uint64_t>(
-            add_lazy(static_cast<int64_t>(uacc1[k]), mont_mul(yi, w1[k], mont), mont.twice_q));
+        wide_mac(wacc0[k], v[k], static_cast<uint64_t>(w0[k]));
+        wide_mac(wacc1[k], v[k], static_cast<uint64_t>(w1[k]));
       }
+    }
+  }
+  if constexpr (!kF64) {
+    // Each sum holds at most 16 products y w with y < 4q (forward outputs)
+    // and w < 2q (key words), so it is below 128 q^2 < 2^127 and its REDC
+    // value r = sum 2^-64 mod q is below 128 q^2 / 2^64 + q < 9q (q < 2^60).
+    // The binary64 rows' REDC radix is 2^62: 4 r is the same residue as
+    // sum 2^-62.  Result in [0, 2q), the inverse butterflies' input range.
+    const uint64_t q = row.q;
+    auto finish = [&](const WideAcc& acc) {
+      uint64_t r = wide_redc64(acc, q, row.neg_qinv64);
+      r = csub(r, 8 * q);
+      r = csub(r, 4 * q);
+      r = csub(r, 2 * q);
+      r = csub(r, q);
+      return csub(r << 2, 2 * q);
+    };
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      uacc0[k] = finish(wacc0[k]);
+      uacc1[k] = finish(wacc1[k]);
     }
   }
   // The first 8 stages of the accumulators' inverse NTT act inside this

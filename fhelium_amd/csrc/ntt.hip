@@ -585,6 +585,8 @@ __global__ void k_prepare_rows(ParamView params, int64_t limbs, NttRow* rows) {
   r.to_mont_p = shoup_companion(r.to_mont, m.q);
   r.ninv_mont_p = shoup_companion(r.ninv_mont, m.q);
   r.ninv_std_p = shoup_companion(r.ninv_std, m.q);
+  // k = -q^{-1} mod 2^62 lifts by one Newton step: q k (2 + q k) = -1 mod 2^124
+  r.neg_qinv64 = m.k * (2 + m.q * m.k);
   rows[l] = r;
 }
 
