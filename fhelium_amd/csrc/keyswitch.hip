@@ -4,7 +4,8 @@
 // (fhelium/engine/hybrid_keyswitch.py:321-371, 530-568 with the native ops
 // csrc/ops/rns/cpu/rns_arithmetic_cpu.cpp:482-700 and
 // csrc/ops/ckks/cpu/ckks_cpu.cpp:474-708), reorganised into eight launches
-// instead of ~100 (ModUp and the tile pass launch once for the binary64 rows
+// (nine for a rotation) instead of ~100 (ModUp and the tile pass launch once
+// for the binary64 rows
 // and once for the integer rows; the integer rows' two launches run on a
 // side stream, concurrently with the binary64 rows' launches):
 //
@@ -27,8 +28,9 @@
 //                         (x - (x mod P)) * P^{-1} with x mod P from the
 //                         mixed-radix digits of the P residues (once per
 //                         coefficient), fused with the engine's final
-//                         canonical addition (optionally through a Galois
-//                         gather of the base).
+//                         canonical addition.  A rotation's base c0 goes
+//                         through the Galois gather first (k_ks_gather_base,
+//                         into the digit workspace, which is dead by then).
 //
 // Exactness: digits are the unique mixed-radix digits; the extended digit is
 // formed with the reference's lazy formula (each term a canonical product,
@@ -1039,9 +1041,7 @@ struct ModDownArgs {
   const int64_t* p_inverse; // [Q0] P^{-1} mod q, standard
   const NttRow* rows;       // global QP NTT records
   View3 base0, base1, out0, out1;
-  int has_base0, has_base1;
-  const int64_t* gal_idx;   // Galois gather of base0 (rotations) or null
-  const int64_t* gal_sign;
+  int has_base0, has_base1;  // a rotation's base0 arrives already gathered
   // Optional fused rescale (null: none): rows 1..lq-1 of the result are
   // written as rows 0..lq-2 of rescale_drop_leading_prime_nearest's output
   // (ckks_cpu.cpp:204-275): ((c_i - c_0) q_0^{-1} + [c_0 > half_drop]) mod q_i.
@@ -1049,6 +1049,27 @@ struct ModDownArgs {
   const int64_t* rescale_inv_shoup;  // [lq - 1] Shoup companions
   int64_t half_drop;
 };
+
+// A rotation's base c0 through the Galois gather, written as contiguous rows
+// for ModDown: out[b, row, i] = +/- c0[b, row, idx[i]] canonicalised, as
+// apply_coefficient_galois_automorphism (ckks_cpu.cpp:390-396) produces it.
+// One XCD walks all coefficient chunks of one (entry, row), so the scattered
+// reads of a source row stay in that XCD's L2; gathered inside ModDown, each
+// XCD's eight-row working set overflowed its L2 and every row was fetched
+// several times.
+__global__ __launch_bounds__(kBlock) void k_ks_gather_base(View3 base, const int64_t* gal_idx,
+                                                          const int64_t* gal_sign, const NttRow* rows,
+                                                          int64_t level, int64_t lq, int64_t batch, int log_n,
+                                                          int64_t* out) {
+  const int64_t n = int64_t{1} << log_n;
+  const uint32_t chunks = static_cast<uint32_t>(n / kBlock);
+  const uint32_t w = xcd_remap(blockIdx.x, chunks * static_cast<uint32_t>(lq * batch));
+  const int64_t i = static_cast<int64_t>(w % chunks) * kBlock + threadIdx.x;
+  const int64_t row = (w / chunks) % lq;
+  const int64_t b = w / (chunks * static_cast<uint32_t>(lq));
+  const int64_t twice_q = static_cast<int64_t>(rows[level + row].twice_q);
+  out[(b * lq + row) * n + i] = read_maybe_galois(base, gal_idx, gal_sign, b, row, i, twice_q);
+}
 
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
@@ -1070,14 +1091,6 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   const bool has_base = (c == 0 ? a.has_base0 : a.has_base1) != 0;
   const View3& out = c == 0 ? a.out0 : a.out1;
   const int64_t row_begin = group * kRows;
-  // Galois gather of the base (c0 of a rotation): one source index and sign
-  // per coefficient, shared by every row.
-  int64_t src_i = i;
-  bool negate = false;
-  if (c == 0 && a.gal_idx != nullptr) {
-    src_i = global(reinterpret_cast<const int32_t*>(a.gal_idx[b]))[i];
-    negate = global(reinterpret_cast<const int8_t*>(a.gal_sign[b]))[i] == -1;
-  }
   // Issue every load of the thread first: P rows, Q rows, base rows.
   int64_t dg[P];
 #pragma unroll
@@ -1087,7 +1100,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   for (int rr = 0; rr < kRows; ++rr) {
     const int64_t row = row_begin + rr;
     xv[rr] = row < a.lq ? x[row * n] : 0;
-    bv[rr] = (has_base && row < a.lq) ? base.at(b, row, src_i) : 0;
+    bv[rr] = (has_base && row < a.lq) ? base.at(b, row, i) : 0;
   }
   // Mixed-radix digits of X mod P (X_P = sum_r c_r prod_{t<r} p_t), each
   // split as c_hi 2^30 + c_lo (both exact in binary64).
@@ -1099,14 +1112,9 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     dhi[r] = static_cast<double>(static_cast<uint32_t>(dg[r] >> 30));
   }
   const int64_t pitch = a.pdig.moddown_stride;
-  // Final canonical addition of the (possibly gathered) base, as
-  // apply_coefficient_galois_automorphism + add_canonical compute it.
+  // Final canonical addition of the base (add_canonical).
   auto with_base = [&](int64_t bval, int64_t value, int64_t twice_q) -> int64_t {
     if (!has_base) return value;
-    if (a.gal_idx != nullptr && c == 0) {
-      if (negate) bval = -bval;
-      bval = canonicalize(shift_positive(bval, twice_q), twice_q);
-    }
     int64_t sum = bval + value;
     if (sum >= twice_q) sum -= twice_q;
     return canonicalize(sum, twice_q);
@@ -1136,7 +1144,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   if (rescale) {
     Global<double>* rec0 = global(a.pdig.moddown_f64) + a.level * pitch;
     const int64_t x0 = row_begin == 0 ? xv[0] : x[0];
-    const int64_t b0 = row_begin == 0 ? bv[0] : (has_base ? base.at(b, 0, src_i) : 0);
+    const int64_t b0 = row_begin == 0 ? bv[0] : (has_base ? base.at(b, 0, i) : 0);
     c0 = with_base(b0, moddown_f64(rec0, x0), 2 * static_cast<int64_t>(static_cast<double>(rec0[14])));
     round_up = c0 > a.half_drop ? 1 : 0;
   }
@@ -1407,8 +1415,17 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   md.out1 = View3(out1);
   md.has_base0 = base0.data != nullptr ? 1 : 0;
   md.has_base1 = base1.data != nullptr ? 1 : 0;
-  md.gal_idx = galois_index_tables;
-  md.gal_sign = galois_sign_tables;
+  if (galois_index_tables != nullptr && base0.data != nullptr) {
+    // The digit rows are dead once both ModUp launches are done (the side
+    // stream joined above): they take the gathered base.
+    const int64_t g_blocks = (n / kBlock) * lq * batch;
+    FH_CHECK_ARG(g_blocks <= 0x7fffffff, "key switch grid too large");
+    hipLaunchKernelGGL(k_ks_gather_base, dim3(static_cast<uint32_t>(g_blocks)), dim3(kBlock), 0, s, View3(base0),
+                       galois_index_tables, galois_sign_tables, static_cast<const NttRow*>(ntt_rows), level, lq,
+                       batch, static_cast<int>(log_n), workspace_digits);
+    FH_CHECK_LAUNCH("key switch base gather");
+    md.base0 = View3(fh_view3{workspace_digits, lq * n, n, 1});
+  }
   FH_CHECK_ARG((rescale_inverse == nullptr) == (rescale_inverse_shoup == nullptr),
                "rescale tables go together");
   FH_CHECK_ARG(rescale_inverse == nullptr || (lq >= 2 && half_drop > 0), "fused rescale needs two Q rows");

@@ -172,7 +172,7 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
 /* A whole hybrid key switch of a batched component in eight launches
  * (decompose, ModUp fused into the forward NTT column pass, tile pass fused
  * with the key MAC over every digit, inverse NTT, ModDown fused with the
- * engine's final canonical addition).  Replaces the per-digit loop of
+ * engine's final canonical addition; a rotation adds one gather of base0).  Replaces the per-digit loop of
  * fhelium/engine/hybrid_keyswitch.py:321-371 plus :530-568 and
  * ckks_engine.py:1664-1667 / :3342, bit for bit; with Galois tables it also
  * replaces the coefficient automorphism of a rotation (ckks_engine.py:3822-3897).
@@ -194,7 +194,8 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
  *                    component and base0 are then read through entry b's
  *                    automorphism (a rotation's key switch)
  *   workspace_*      digits [batch, lq, N], inter [batch, digit_count, lqp, N],
- *                    acc [2, batch, lqp, N]
+ *                    acc [2, batch, lqp, N]; with Galois tables the digit
+ *                    workspace finally holds the gathered base0 rows
  *   base0 / base1    optional canonical rows added to correction 0 / 1
  *                    (data == NULL: none); out0 / out1 receive the results.
  *   rescale_inverse  NULL, or [lq - 1] q_level^{-1} mod q_{level+i} (standard)
