@@ -105,27 +105,6 @@ struct KsArgs {
   int64_t small_rows;       // leading QP rows on the binary64 path
 };
 
-// With fast arithmetic the binary64 rows of the ModUp output travel packed:
-// canonical residues (< 2^41) as a u32 plane and a u16 plane, 6 bytes per
-// coefficient instead of 8.  In the [Lqp, N] int64 slab of one (entry,
-// digit) the u32 plane starts at the slab and the u16 plane follows it; both
-// fit inside the slab's first small_rows rows, so the integer rows keep
-// their int64 layout after them.  Within a row the coefficients are stored
-// in 8 x 16 blocks of the 2^(logN-8) x 256 matrix (row i, column c): a band
-// of 8 rows is the 8 tiles one MAC block reads, and each 16-column block
-// row is written by one ModUp wave, so both kernels move whole cache lines.
-struct PackedInter {
-  uint32_t* lo;
-  uint16_t* hi;
-};
-__device__ __forceinline__ PackedInter packed_slab(int64_t* slab, int64_t small_rows, int64_t n) {
-  uint32_t* lo = reinterpret_cast<uint32_t*>(slab);
-  return PackedInter{lo, reinterpret_cast<uint16_t*>(lo + small_rows * n)};
-}
-__device__ __forceinline__ int64_t packed_index(int64_t i, int64_t c) {
-  return ((i >> 3) << 11) | ((c >> 4) << 7) | ((i & 7) << 4) | (c & 15);
-}
-
 // Block i runs on XCD i % 8; give each XCD one contiguous range of work
 // items (batch fastest) so consecutive items on an XCD share key tiles in
 // its L2.  The map is a bijection of [0, total).
@@ -699,7 +678,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
   }
   __syncthreads();
   auto inv_slot = [](int st, int e) { return (kTile - (kTile >> st)) + (e >> (st + 1)); };
-  auto inverse_tile = [&](const auto& pi, auto (&v)[8], int64_t* out) {
+  auto inverse_tile = [&](const auto& pi, auto (&v)[8], auto store) {
     // Round 1 (slots = bits 2..0 of the element): inverse stages 0..2.
 #pragma unroll
     for (int st = 0; st < 3; ++st) {
@@ -744,20 +723,28 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) out[mac_e1(t, k)] = pi.mid(v[k]);
+    for (int k = 0; k < 8; ++k) store(mac_e1(t, k), v[k]);
     // the exchange buffer is reused by the next accumulator
     __syncthreads();
   };
-  int64_t* out0 = a.acc + ((0 * a.batch + b) * a.lqp + j) * n + tile_off;
-  int64_t* out1 = a.acc + ((1 * a.batch + b) * a.lqp + j) * n + tile_off;
   if constexpr (kF64) {
+    // binary64 rows of the accumulators travel packed (canonical, 6 bytes)
     const F64Fast pi(row);
-    inverse_tile(pi, facc0, out0);
-    inverse_tile(pi, facc1, out1);
+    const PackedInter pk0 = packed_slab(a.acc + (0 * a.batch + b) * a.lqp * n, a.small_rows, n);
+    const PackedInter pk1 = packed_slab(a.acc + (1 * a.batch + b) * a.lqp * n, a.small_rows, n);
+    const int64_t row_off = j * n;
+    inverse_tile(pi, facc0, [&](int e, double x) {
+      packed_store(pk0, row_off + packed_index(r, e), f64_to_u64(pi.canon(x)));
+    });
+    inverse_tile(pi, facc1, [&](int e, double x) {
+      packed_store(pk1, row_off + packed_index(r, e), f64_to_u64(pi.canon(x)));
+    });
   } else {
     const U64Fast pi(row);
-    inverse_tile(pi, uacc0, out0);
-    inverse_tile(pi, uacc1, out1);
+    int64_t* out0 = a.acc + ((0 * a.batch + b) * a.lqp + j) * n + tile_off;
+    int64_t* out1 = a.acc + ((1 * a.batch + b) * a.lqp + j) * n + tile_off;
+    inverse_tile(pi, uacc0, [&](int e, uint64_t x) { out0[e] = static_cast<int64_t>(x); });
+    inverse_tile(pi, uacc1, [&](int e, uint64_t x) { out1[e] = static_cast<int64_t>(x); });
   }
 }
 
@@ -1033,8 +1020,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSmall ?
 //    rows hold canonical standard residues of X over Q_l P.
 // ---------------------------------------------------------------------------
 struct ModDownArgs {
-  const int64_t* acc;       // [2, B, Lqp, N]
-  int64_t batch, lq, lqp, level;
+  const int64_t* acc;       // [2, B, Lqp, N], rows [0, small_rows) packed
+  int64_t batch, lq, lqp, level, small_rows;
   int log_n;
   ParamView params;
   KsDigit pdig;             // the P primes as one digit (extension over QP0 columns)
@@ -1081,12 +1068,22 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   const uint32_t groups = static_cast<uint32_t>((a.lq + kRows - 1) / kRows);
   const uint32_t batch = static_cast<uint32_t>(a.batch);
   const uint32_t w = xcd_remap(blockIdx.x, chunks * groups * 2 * batch);
-  const int64_t i = static_cast<int64_t>(w % chunks) * kBlock + threadIdx.x;
+  // Threads walk packed positions: the packed rows are then read as whole
+  // runs, and each wave's 64 coefficients are four runs of 16 consecutive
+  // ones (whole cache lines of the int64 rows).
+  const int64_t pidx = static_cast<int64_t>(w % chunks) * kBlock + threadIdx.x;
+  const int64_t i = packed_element(pidx);
   const int64_t group = (w / chunks) % groups;
   const uint32_t cb = w / (chunks * groups);
   const int64_t c = cb / batch;
   const int64_t b = cb % batch;
-  const int64_t* x = a.acc + (c * a.batch + b) * a.lqp * n + i;
+  int64_t* slab = const_cast<int64_t*>(a.acc) + (c * a.batch + b) * a.lqp * n;
+  const int64_t* x = slab + i;
+  // Q rows below small_rows are packed binary64 rows (canonical residues)
+  const PackedInter pk = packed_slab(slab, a.small_rows, n);
+  auto acc_row = [&](int64_t row) -> int64_t {
+    return row < a.small_rows ? static_cast<int64_t>(packed_load(pk, row * n + pidx)) : x[row * n];
+  };
   const View3& base = c == 0 ? a.base0 : a.base1;
   const bool has_base = (c == 0 ? a.has_base0 : a.has_base1) != 0;
   const View3& out = c == 0 ? a.out0 : a.out1;
@@ -1094,12 +1091,12 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   // Issue every load of the thread first: P rows, Q rows, base rows.
   int64_t dg[P];
 #pragma unroll
-  for (int r = 0; r < P; ++r) dg[r] = x[(a.lq + r) * n];
+  for (int r = 0; r < P; ++r) dg[r] = acc_row(a.lq + r);
   int64_t xv[kRows], bv[kRows];
 #pragma unroll
   for (int rr = 0; rr < kRows; ++rr) {
     const int64_t row = row_begin + rr;
-    xv[rr] = row < a.lq ? x[row * n] : 0;
+    xv[rr] = row < a.lq ? acc_row(row) : 0;
     bv[rr] = (has_base && row < a.lq) ? base.at(b, row, i) : 0;
   }
   // Mixed-radix digits of X mod P (X_P = sum_r c_r prod_{t<r} p_t), each
@@ -1143,7 +1140,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   int64_t round_up = 0;
   if (rescale) {
     Global<double>* rec0 = global(a.pdig.moddown_f64) + a.level * pitch;
-    const int64_t x0 = row_begin == 0 ? xv[0] : x[0];
+    const int64_t x0 = row_begin == 0 ? xv[0] : acc_row(0);
     const int64_t b0 = row_begin == 0 ? bv[0] : (has_base ? base.at(b, 0, i) : 0);
     c0 = with_base(b0, moddown_f64(rec0, x0), 2 * static_cast<int64_t>(static_cast<double>(rec0[14])));
     round_up = c0 > a.half_drop ? 1 : 0;
@@ -1395,7 +1392,7 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   // The tile stages of this inverse ran inside the MAC kernel.
   int st = ntt::inverse_columns_internal(workspace_acc, 2 * batch, lqp, static_cast<int>(log_n), lqp * n, n,
                                          static_cast<const NttRow*>(ntt_rows), inverse_twiddles,
-                                         inverse_twiddles_shoup, level, kOutStdMac, s);
+                                         inverse_twiddles_shoup, level, kOutStdMac, small_rows, s);
   if (st != kOk) return st;
 
   ModDownArgs md;
@@ -1404,6 +1401,7 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   md.lq = lq;
   md.lqp = lqp;
   md.level = level;
+  md.small_rows = small_rows;
   md.log_n = static_cast<int>(log_n);
   md.params = a.params;
   md.rows = static_cast<const NttRow*>(ntt_rows);
@@ -1518,5 +1516,5 @@ FH_API int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, 
     FH_CHECK_LAUNCH("fused tensor product tiles (integer rows)");
   }
   return ntt::inverse_columns_internal(out, 3 * batch, limbs, static_cast<int>(log_n), limbs * n, n, rows,
-                                       inverse_twiddles, inverse_twiddles_shoup, row_start, kOutStdMac, s);
+                                       inverse_twiddles, inverse_twiddles_shoup, row_start, kOutStdMac, 0, s);
 }

@@ -202,9 +202,10 @@ __device__ __forceinline__ void inv_tiles(const P& p, int log_n, int64_t* base, 
 
 // Inverse column pass: stages 8 .. L-1 pairing row bits 0 .. S1-1, then the
 // N^{-1} normalisation and output representation.  Reads intermediates,
-// writes output residues.
-template <int S1, class P>
-__device__ __forceinline__ void inv_columns(const P& p, int64_t* base, const int64_t* tw,
+// writes output residues through load(i, c) / store(i, c, value) of element
+// i * 256 + c of the row.
+template <int S1, class P, class Load, class Store>
+__device__ __forceinline__ void inv_columns(const P& p, Load load, Store store, const int64_t* tw,
                                             const int64_t* twp, int out, int64_t* lds) {
   constexpr int R1 = 1 << S1;
   constexpr int T = R1 / 16;
@@ -216,7 +217,7 @@ __device__ __forceinline__ void inv_columns(const P& p, int64_t* base, const int
   const int col = blockIdx.x * C + c;
   V v[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = p.unmid(base[static_cast<int64_t>(16 * t + k) * kTile + col]);
+  for (int k = 0; k < 16; ++k) v[k] = p.unmid(load(16 * t + k, col));
   // Round A: row bits 0..3 (bits of k); stage s = 8 + a.
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -246,12 +247,10 @@ __device__ __forceinline__ void inv_columns(const P& p, int64_t* base, const int
       }
     }
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
-      base[static_cast<int64_t>(t + T * k) * kTile + col] = p.out_inv(v[k], out);
+    for (int k = 0; k < 16; ++k) store(t + T * k, col, p.out_inv(v[k], out));
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
-      base[static_cast<int64_t>(16 * t + k) * kTile + col] = p.out_inv(v[k], out);
+    for (int k = 0; k < 16; ++k) store(16 * t + k, col, p.out_inv(v[k], out));
   }
 }
 
@@ -271,6 +270,7 @@ struct NttArgs {
   const int64_t* src;   // input rows of the first pass (null: in place)
   int64_t src_batch_stride;
   int64_t src_limb_stride;
+  int64_t packed_rows;  // inverse column pass: leading packed binary64 limbs
 };
 
 struct RowRef {
@@ -279,6 +279,8 @@ struct RowRef {
   const int64_t* tw;
   const int64_t* twp;
   const int64_t* src;   // input row of the first pass
+  int64_t* entry;       // limb 0 of the row's batch entry
+  int64_t limb;
 };
 
 __device__ __forceinline__ RowRef row_ref(const NttArgs& a) {
@@ -291,7 +293,7 @@ __device__ __forceinline__ RowRef row_ref(const NttArgs& a) {
   const int64_t n = int64_t{1} << a.log_n;
   int64_t* base = a.data + b * a.batch_stride + l * a.limb_stride;
   const int64_t* src = a.src ? a.src + b * a.src_batch_stride + l * a.src_limb_stride : base;
-  return RowRef{base, a.rows + prow, a.tw + prow * n, a.twp + prow * n, src};
+  return RowRef{base, a.rows + prow, a.tw + prow * n, a.twp + prow * n, src, a.data + b * a.batch_stride, l};
 }
 
 enum Mode : int { kExact = 0, kFast = 1 };
@@ -343,7 +345,22 @@ template <int S1, int MODE>
 __global__ __launch_bounds__(kBlock) void k_inv_columns(NttArgs a) {
   __shared__ int64_t lds[(1 << S1) * (kBlock / ((1 << S1) / 16))];
   const RowRef rr = row_ref(a);
-  FH_DISPATCH_PATH(MODE, *rr.row, (inv_columns<S1>(p, rr.base, rr.tw, rr.twp, a.form, lds)));
+  if (rr.limb < a.packed_rows) {
+    // packed binary64 limb (fast arithmetic; input and output in place)
+    const int64_t n = int64_t{1} << a.log_n;
+    const PackedInter pk = packed_slab(rr.entry, a.packed_rows, n);
+    const int64_t off = rr.limb * n;
+    auto load = [&](int i, int c) -> int64_t {
+      return __double_as_longlong(u64_to_f64(packed_load(pk, off + packed_index(i, c))));
+    };
+    auto store = [&](int i, int c, int64_t v) { packed_store(pk, off + packed_index(i, c), static_cast<uint64_t>(v)); };
+    inv_columns<S1>(F64Fast(*rr.row), load, store, rr.tw, rr.twp, a.form, lds);
+    return;
+  }
+  int64_t* base = rr.base;
+  auto load = [&](int i, int c) -> int64_t { return base[static_cast<int64_t>(i) * kTile + c]; };
+  auto store = [&](int i, int c, int64_t v) { base[static_cast<int64_t>(i) * kTile + c] = v; };
+  FH_DISPATCH_PATH(MODE, *rr.row, (inv_columns<S1>(p, load, store, rr.tw, rr.twp, a.form, lds)));
 }
 
 // ---------------------------------------------------------------------------
@@ -651,9 +668,12 @@ int forward_columns_internal(int64_t* dst, const int64_t* src, int64_t batch, in
 int inverse_columns_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n,
                              int64_t batch_stride, int64_t limb_stride, const NttRow* rows,
                              const int64_t* tw, const int64_t* twp, int64_t row_start,
-                             int output_form, hipStream_t stream) {
+                             int output_form, int64_t packed_rows, hipStream_t stream) {
   FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "column-only inverse needs logN 12..16");
+  FH_CHECK_ARG(packed_rows >= 0 && packed_rows <= limbs && (packed_rows == 0 || limb_stride == (int64_t{1} << log_n)),
+               "packed rows need contiguous limbs");
   NttArgs a{data, batch_stride, limb_stride, limbs, 0, rows, tw, twp, row_start, log_n, output_form, 1};
+  a.packed_rows = packed_rows;
   const int s1 = log_n - kTileLog;
   const int T = (1 << s1) / 16;
   const int C = kBlock / T;

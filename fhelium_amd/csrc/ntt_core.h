@@ -243,6 +243,43 @@ struct U64Fast {
   }
 };
 
+// Packed binary64 rows.  With fast arithmetic the binary64 rows of the key
+// switch's intermediates (the ModUp output and the accumulators) travel as
+// canonical residues (< 2^46) in a u32 plane and a u16 plane, 6 bytes per
+// coefficient instead of 8.  In the [rows, N] int64 slab of one (entry,
+// digit) or (component, entry) the u32 plane starts at the slab and the u16
+// plane follows it; both fit inside the slab's first small_rows rows, so the
+// integer rows keep their int64 layout after them.  Within a row the
+// coefficients are stored in 8 x 16 blocks of the 2^(logN-8) x 256 matrix
+// (row i = tile, column c): a band of 8 rows is the 8 tiles one tile-pass
+// block covers, and one 16-column block row is what a column-pass wave
+// covers, so both passes move whole cache lines.
+struct PackedInter {
+  uint32_t* lo;
+  uint16_t* hi;
+};
+__device__ __forceinline__ PackedInter packed_slab(int64_t* slab, int64_t small_rows, int64_t n) {
+  uint32_t* lo = reinterpret_cast<uint32_t*>(slab);
+  return PackedInter{lo, reinterpret_cast<uint16_t*>(lo + small_rows * n)};
+}
+__device__ __forceinline__ int64_t packed_index(int64_t i, int64_t c) {
+  return ((i >> 3) << 11) | ((c >> 4) << 7) | ((i & 7) << 4) | (c & 15);
+}
+// Inverse of packed_index: the coefficient (i * 256 + c) at packed position p.
+__device__ __forceinline__ int64_t packed_element(int64_t p) {
+  const int64_t i = ((p >> 11) << 3) | ((p >> 4) & 7);
+  const int64_t c = (((p >> 7) & 15) << 4) | (p & 15);
+  return (i << 8) | c;
+}
+// Element idx (row offset plus packed_index) of a packed slab.
+__device__ __forceinline__ uint64_t packed_load(const PackedInter& pk, int64_t idx) {
+  return static_cast<uint64_t>(pk.lo[idx]) | static_cast<uint64_t>(pk.hi[idx]) << 32;
+}
+__device__ __forceinline__ void packed_store(const PackedInter& pk, int64_t idx, uint64_t v) {
+  pk.lo[idx] = static_cast<uint32_t>(v);
+  pk.hi[idx] = static_cast<uint16_t>(v >> 32);
+}
+
 // Inverse transform launcher shared with keyswitch.hip (fast arithmetic).
 // With in_mid the input rows hold working-format values (binary64 bits for
 // rows below 2^46, u64 otherwise) instead of int64 residues.
@@ -257,10 +294,12 @@ int forward_columns_internal(int64_t* dst, const int64_t* src, int64_t batch, in
                              const int64_t* twp, int64_t row_start, bool to_montgomery, hipStream_t stream);
 // Only the column pass of that inverse (logN 12..16): for rows whose tile
 // stages were already applied, holding working-format values.
+// packed_rows > 0: limbs [0, packed_rows) of each batch entry are binary64
+// rows held packed (PackedInter over the entry's slab, input and output).
 int inverse_columns_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n,
                              int64_t batch_stride, int64_t limb_stride, const NttRow* rows,
                              const int64_t* tw, const int64_t* twp, int64_t row_start,
-                             int output_form, hipStream_t stream);
+                             int output_form, int64_t packed_rows, hipStream_t stream);
 
 }  // namespace ntt
 }  // namespace fh
