@@ -74,8 +74,11 @@ struct KsDigit {
   const int64_t* basis_std;    // [D-1, QP0] M_r R mod p_j
   int64_t f64_ok;              // every source prime < 2^45: digits stored as binary64
   const int64_t* basis_shoup;  // [D-1, QP0] Shoup companions of basis_std
-  const double* moddown_f64;   // P digit only: [Q0, 16] binary64 ModDown records
-  int64_t moddown_stride;      // record pitch (16 words)
+  // P digit: [Q0, 16] binary64 ModDown records (f64_stride = 16).  A Q digit
+  // with f64_ok and D > 1: its binary64 Garner record [D-1 normalizers,
+  // (D-1) x D propagation, D moduli, D reciprocals], standard form.
+  const double* f64_table;
+  int64_t f64_stride;
 };
 static_assert(sizeof(KsDigit) == 16 * 8, "KsDigit must stay 16 words");
 
@@ -206,6 +209,35 @@ __device__ __forceinline__ void garner_fixed(const KsDigit& d, int64_t* dg) {
   }
 }
 
+// The same recurrence in binary64 for a digit whose primes are all below
+// 2^45, from its Garner record (standard-form constants).  Each product is
+// f64_mulmod_signed, exact and within (-0.53q, 0.53q) (the operands stay
+// below 2^47); running sums of at most R such terms plus a canonical value
+// are exact.  Each digit is corrected into [0, q) once, so the digits are
+// the unique mixed-radix digits, as the integer recurrence produces them.
+template <int R>
+__device__ __forceinline__ void garner_f64(Global<double>* rec, const int64_t* x, double* dg) {
+  Global<double>* norm = rec;
+  Global<double>* prop = rec + (R - 1);
+  Global<double>* qs = prop + (R - 1) * R;
+  Global<double>* qinv = qs + R;
+  double first = static_cast<double>(x[0]);
+  first = first < qs[0] ? first : first - qs[0];
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = first;
+  dg[0] = first;
+#pragma unroll
+  for (int row = 1; row < R; ++row) {
+    const double q = qs[row];
+    double digit = f64_mulmod_signed(norm[row - 1], static_cast<double>(x[row]) - acc[row], q, qinv[row]);
+    digit = digit < 0.0 ? digit + q : digit;
+    dg[row] = digit;
+#pragma unroll
+    for (int t = row + 1; t < R; ++t) acc[t] += f64_mulmod_signed(prop[(row - 1) * R + t], digit, qs[t], qinv[t]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 1. Garner digits.  Grid (N/256, dnum, B).
 // ---------------------------------------------------------------------------
@@ -223,12 +255,22 @@ __device__ __forceinline__ void decompose_one(const KsArgs& a, const KsDigit& d,
                                          a.params.at(kTwiceModulus, a.level + d.row_begin + r))
                      : 0;
   }
+  int64_t* out = a.digits + (b * a.lq + d.row_begin) * n + i;
+  if constexpr (R >= 2) {
+    if (d.f64_table != nullptr) {
+      // binary64 digit (f64_ok): Garner in binary64, stored as binary64
+      double fd[R];
+      garner_f64<R>(global(d.f64_table), dg, fd);
+#pragma unroll
+      for (int r = 0; r < R; ++r) out[static_cast<int64_t>(r) * n] = __double_as_longlong(fd[r]);
+      return;
+    }
+  }
   if constexpr (R == 0) {
     garner(d, dg);
   } else {
     garner_fixed<R>(d, dg);
   }
-  int64_t* out = a.digits + (b * a.lq + d.row_begin) * n + i;
   // Digits below 2^45 are stored as binary64 (exact), the form the ModUp
   // kernel multiplies in; the others stay integers.
   const bool f64 = d.f64_ok != 0;
@@ -1108,7 +1150,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     dlo[r] = static_cast<double>(static_cast<uint32_t>(dg[r] & ((int64_t{1} << 30) - 1)));
     dhi[r] = static_cast<double>(static_cast<uint32_t>(dg[r] >> 30));
   }
-  const int64_t pitch = a.pdig.moddown_stride;
+  const int64_t pitch = a.pdig.f64_stride;
   // Final canonical addition of the base (add_canonical).
   auto with_base = [&](int64_t bval, int64_t value, int64_t twice_q) -> int64_t {
     if (!has_base) return value;
@@ -1139,7 +1181,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   int64_t c0 = 0;
   int64_t round_up = 0;
   if (rescale) {
-    Global<double>* rec0 = global(a.pdig.moddown_f64) + a.level * pitch;
+    Global<double>* rec0 = global(a.pdig.f64_table) + a.level * pitch;
     const int64_t x0 = row_begin == 0 ? xv[0] : acc_row(0);
     const int64_t b0 = row_begin == 0 ? bv[0] : (has_base ? base.at(b, 0, i) : 0);
     c0 = with_base(b0, moddown_f64(rec0, x0), 2 * static_cast<int64_t>(static_cast<double>(rec0[14])));
@@ -1150,7 +1192,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     const int64_t row = row_begin + rr;
     if (row >= a.lq) break;
     const int64_t grow = a.level + row;
-    Global<double>* rec = global(a.pdig.moddown_f64) + grow * pitch;
+    Global<double>* rec = global(a.pdig.f64_table) + grow * pitch;
     const double q = rec[14];
     int64_t value;
     if (q != 0.0) {

@@ -16,6 +16,7 @@ import torch
 
 from fhelium_amd import kernels as K
 from fhelium_amd.core import KeySwitchKey
+from fhelium_amd.rns import F64_DIGIT_LIMIT
 from fhelium_amd.runtime import RnsRuntime
 
 
@@ -53,7 +54,15 @@ class HybridKeySwitcher:
         d = len(t.prime_ids)
         multi = d > 1
         # binary64 basis extension needs every digit below 2^45
-        f64_ok = int(all(self.config.moduli[i] < (1 << 45) for i in t.prime_ids))
+        f64_ok = int(all(self.config.moduli[i] < F64_DIGIT_LIMIT for i in t.prime_ids))
+        # words 14, 15: the P digit's ModDown records, or a Q digit's
+        # binary64 Garner record (f64_ok, D > 1)
+        if moddown_table is not None:
+            f64_words = [moddown_table.data_ptr(), moddown_table.stride(0)]
+        elif t.garner_f64.numel():
+            f64_words = [t.garner_f64.data_ptr(), 0]
+        else:
+            f64_words = [0, 0]
         return [
             row_begin, d, key_digit,
             t.normalizers.data_ptr() if multi else 0,
@@ -65,8 +74,7 @@ class HybridKeySwitcher:
             t.basis_extension_std.data_ptr() if multi else 0,
             f64_ok,
             t.basis_extension_shoup.data_ptr() if multi else 0,
-            moddown_table.data_ptr() if moddown_table is not None else 0,
-            moddown_table.stride(0) if moddown_table is not None else 0,
+            *f64_words,
         ]
 
     def p_digit_table(self) -> torch.Tensor:

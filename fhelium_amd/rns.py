@@ -130,6 +130,14 @@ class DigitTables:
     basis_extension: torch.Tensor      # [D-1, QP0]  M_r R^2 mod p_j over level-0 QP rows
     basis_extension_std: torch.Tensor  # [D-1, QP0]  M_r R mod p_j (REDC of the above)
     basis_extension_shoup: torch.Tensor  # [D-1, QP0] floor(std * 2^64 / p_j) as two's complement
+    # binary64 Garner record when every prime is below 2^45 (else empty):
+    # [D-1 normalizers, (D-1) x D propagation, D moduli, D reciprocals], the
+    # constants in standard form (the tables above without their R factor)
+    garner_f64: torch.Tensor
+
+
+# Digits whose primes are all below this are carried as binary64 (exact).
+F64_DIGIT_LIMIT = 1 << 45
 
 
 def _digit_tables(moduli: tuple[int, ...], prime_ids: tuple[int, ...], qp_ids: tuple[int, ...]) -> DigitTables:
@@ -150,6 +158,11 @@ def _digit_tables(moduli: tuple[int, ...], prime_ids: tuple[int, ...], qp_ids: t
     basis_std = [[(products[c] * R) % dest[j] for j in range(len(dest))] for c in range(d - 1)]
     basis_shoup = [[_as_signed64((w << 64) // dest[j]) for j, w in enumerate(row)] for row in basis_std]
     ks = [(-pow(q, -1, R)) % R for q in src]
+    garner = []
+    if d > 1 and all(q < F64_DIGIT_LIMIT for q in src):
+        garner += [float(pow(products[c], -1, src[c + 1])) for c in range(d - 1)]
+        garner += [float(products[c] % src[t]) if t >= c + 2 else 0.0 for c in range(d - 1) for t in range(d)]
+        garner += [float(q) for q in src] + [1.0 / q for q in src]
     as_t = lambda v, shape=None: torch.tensor(v, dtype=torch.int64).reshape(shape) if shape else torch.tensor(v, dtype=torch.int64)
     return DigitTables(
         prime_ids=prime_ids,
@@ -164,6 +177,7 @@ def _digit_tables(moduli: tuple[int, ...], prime_ids: tuple[int, ...], qp_ids: t
                              else torch.empty(0, len(dest), dtype=torch.int64)),
         basis_extension_shoup=(as_t(basis_shoup, (d - 1, len(dest))) if d > 1
                                else torch.empty(0, len(dest), dtype=torch.int64)),
+        garner_f64=torch.tensor(garner, dtype=torch.float64),
     )
 
 

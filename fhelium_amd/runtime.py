@@ -32,13 +32,14 @@ class DeviceDigitTables:
     basis_extension: torch.Tensor  # [D-1, QP0]
     basis_extension_std: torch.Tensor  # [D-1, QP0]
     basis_extension_shoup: torch.Tensor  # [D-1, QP0]
+    garner_f64: torch.Tensor  # binary64 Garner record or empty
 
     @classmethod
     def from_host(cls, key, t, dev) -> "DeviceDigitTables":
         return cls(key, t.normalizers.to(dev), t.propagation.to(dev).contiguous(), t.modulus_lo.to(dev),
                    t.modulus_hi.to(dev), t.neg_inv_lo.to(dev), t.neg_inv_hi.to(dev),
                    t.basis_extension.to(dev).contiguous(), t.basis_extension_std.to(dev).contiguous(),
-                   t.basis_extension_shoup.to(dev).contiguous())
+                   t.basis_extension_shoup.to(dev).contiguous(), t.garner_f64.to(dev))
 
 
 class RnsRuntime:
