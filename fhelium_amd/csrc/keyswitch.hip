@@ -534,6 +534,17 @@ constexpr int kMacTiles = kBlock / kMacThreads;     // tiles per block
 constexpr int kMacExchange = kMacTiles * kTilePitch;
 constexpr int kMacMaxDigits = 16;
 
+// The 32 threads of one tile are half of one wave64, and each tile's LDS
+// exchange buffer and twiddles are its own: an exchange inside a tile needs
+// only wave-level ordering (LDS operations of one wave complete in order),
+// not a barrier over the block's four waves.
+static_assert(64 % kMacThreads == 0, "a tile's threads must lie in one wave");
+__device__ __forceinline__ void tile_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Element of the tile held in slot k by thread t in each round.
 __device__ __forceinline__ int mac_e1(int t, int k) { return t + 32 * k; }
 __device__ __forceinline__ int mac_e2(int t, int k) { return ((t >> 2) << 5) | (k << 2) | (t & 3); }
@@ -603,13 +614,13 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       const uint4 hh = *reinterpret_cast<const uint4*>(pk.hi + idx);
       const uint32_t lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
       const uint32_t hi2[4] = {hh.x, hh.y, hh.z, hh.w};
-      __syncthreads();  // the previous digit's last reads of the exchange buffer are done
+      tile_sync();  // the previous digit's last reads of the exchange buffer are done
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const uint64_t h = (hi2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
         sh[tile_pad(mac_e3(t, k))] = __double_as_longlong(u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32)));
       }
-      __syncthreads();
+      tile_sync();
 #pragma unroll
       for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
     } else {
@@ -636,10 +647,10 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       }
     }
     // The previous digit's last reads of this buffer must be done.
-    __syncthreads();
+    tile_sync();
 #pragma unroll
     for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
-    __syncthreads();
+    tile_sync();
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
     // Round 2: tile stages 3..5 pair element bits 4..2 = bits 2..0 of k.
@@ -655,7 +666,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     // Each thread rewrites the words it just read: no barrier needed.
 #pragma unroll
     for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
-    __syncthreads();
+    tile_sync();
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
     // Round 3: tile stages 6..7 pair element bits 1..0 = bits 1..0 of k.
@@ -705,7 +716,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
   // The first 8 stages of the accumulators' inverse NTT act inside this
   // tile: run them here (fast arithmetic, as the rest of the inverse) and
   // write the working-format result; only the column pass remains.
-  __syncthreads();
+  tile_sync();
   {
     const int64_t* w_row = a.inv_w + grow * n;
     const int64_t* wp_row = a.inv_wp + grow * n;
@@ -718,7 +729,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       if constexpr (!kF64) twp[idx] = wp_row[gi];
     }
   }
-  __syncthreads();
+  tile_sync();
   auto inv_slot = [](int st, int e) { return (kTile - (kTile >> st)) + (e >> (st + 1)); };
   auto inverse_tile = [&](const auto& pi, auto (&v)[8], auto store) {
     // Round 1 (slots = bits 2..0 of the element): inverse stages 0..2.
@@ -734,7 +745,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e3(t, k))] = pi.mid(v[k]);
-    __syncthreads();
+    tile_sync();
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = pi.unmid(sh[tile_pad(mac_e2(t, k))]);
     // Round 2 (slots = bits 4..2): stages 3, 4.
@@ -750,7 +761,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = pi.mid(v[k]);
-    __syncthreads();
+    tile_sync();
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = pi.unmid(sh[tile_pad(mac_e1(t, k))]);
     // Round 3 (slots = bits 7..5): stages 5..7.
