@@ -534,16 +534,10 @@ constexpr int kMacTiles = kBlock / kMacThreads;     // tiles per block
 constexpr int kMacExchange = kMacTiles * kTilePitch;
 constexpr int kMacMaxDigits = 16;
 
-// The 32 threads of one tile are half of one wave64, and each tile's LDS
-// exchange buffer and twiddles are its own: an exchange inside a tile needs
-// only wave-level ordering (LDS operations of one wave complete in order),
-// not a barrier over the block's four waves.
+// The 32 threads of one tile are half of one wave64 and each tile's LDS
+// exchange buffer and twiddles are its own, so in-tile exchanges use
+// tile_sync() rather than a block barrier.
 static_assert(64 % kMacThreads == 0, "a tile's threads must lie in one wave");
-__device__ __forceinline__ void tile_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Element of the tile held in slot k by thread t in each round.
 __device__ __forceinline__ int mac_e1(int t, int k) { return t + 32 * k; }
@@ -882,7 +876,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
       if constexpr (!kF64) twp[idx] = wp_row[gi];
     }
   }
-  __syncthreads();
+  tile_sync();
   auto tw_at = [&](int st, int e) {
     const int slot = mac_tw_slot(st, e);
     return p.tw_make(tww[slot], kF64 ? 0 : twp[slot]);
@@ -905,10 +899,10 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
         p.fwd(v[k], v[k + half], tw_at(st, mac_e1(t, k)));
       }
     }
-    __syncthreads();  // earlier readers of the exchange buffer are done
+    tile_sync();  // earlier readers of the exchange buffer are done
 #pragma unroll
     for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
-    __syncthreads();
+    tile_sync();
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
 #pragma unroll
@@ -922,7 +916,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
-    __syncthreads();
+    tile_sync();
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
 #pragma unroll
@@ -983,7 +977,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
     }
   }
   // Inverse tile stages (the first eight of the inverse NTT) of d0, d1, d2.
-  __syncthreads();  // every thread is done with the forward twiddles
+  tile_sync();  // every thread is done with the forward twiddles
   {
     const int64_t* w_row = a.inv_w + grow * n;
     const int64_t* wp_row = a.inv_wp + grow * n;
@@ -995,7 +989,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
       if constexpr (!kF64) twp[idx] = wp_row[gi];
     }
   }
-  __syncthreads();
+  tile_sync();
   auto inv_slot = [](int st, int e) { return (kTile - (kTile >> st)) + (e >> (st + 1)); };
   auto inverse_tile = [&](V (&v)[8], int64_t* out) {
 #pragma unroll
@@ -1010,7 +1004,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e3(t, k))] = p.mid(v[k]);
-    __syncthreads();
+    tile_sync();
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
 #pragma unroll
@@ -1025,7 +1019,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
-    __syncthreads();
+    tile_sync();
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
 #pragma unroll
@@ -1040,7 +1034,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) out[mac_e1(t, k)] = p.mid(v[k]);
-    __syncthreads();  // the exchange buffer is reused by the next product
+    tile_sync();  // the exchange buffer is reused by the next product
   };
   int64_t* out = a.out + (b * a.limbs + j) * n + tile_off;
   inverse_tile(d0, out);

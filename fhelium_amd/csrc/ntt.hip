@@ -128,7 +128,7 @@ __device__ __forceinline__ void fwd_tiles(const P& p, int log_n, int64_t* base, 
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) sh[tile_pad(t + 16 * k)] = p.mid(v[k]);
-  __syncthreads();
+  tile_sync();
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = p.unmid(sh[17 * t + k]);
 #pragma unroll
@@ -145,10 +145,12 @@ __device__ __forceinline__ void fwd_tiles(const P& p, int log_n, int64_t* base, 
   // Each thread rewrites exactly the words it just read: no barrier needed.
 #pragma unroll
   for (int k = 0; k < 16; ++k) sh[17 * t + k] = p.out_fwd(v[k]);
-  __syncthreads();
+  tile_sync();
 #pragma unroll
   for (int k = 0; k < 16; ++k) tile[t + 16 * k] = sh[tile_pad(t + 16 * k)];
 }
+
+static_assert(64 % (kBlock / kTilesPerBlock) == 0, "a tile's threads must lie in one wave");
 
 // Inverse tile pass: stages 0..7 inside 256-element tiles.  Reads input
 // residues, writes intermediates.
@@ -166,7 +168,7 @@ __device__ __forceinline__ void inv_tiles(const P& p, int log_n, int64_t* base, 
   const int n = 1 << log_n;
 #pragma unroll
   for (int k = 0; k < 16; ++k) sh[tile_pad(t + 16 * k)] = in_tile[t + 16 * k];
-  __syncthreads();
+  tile_sync();
   V v[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = in_mid ? p.unmid(sh[17 * t + k]) : p.in(sh[17 * t + k]);
@@ -183,7 +185,7 @@ __device__ __forceinline__ void inv_tiles(const P& p, int log_n, int64_t* base, 
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) sh[17 * t + k] = p.mid(v[k]);
-  __syncthreads();
+  tile_sync();
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = p.unmid(sh[tile_pad(t + 16 * k)]);
   // Round B: stages 4..7 pair tile bits 4..7 (bits of k).

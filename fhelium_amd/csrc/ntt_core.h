@@ -16,6 +16,16 @@ constexpr int kTilePitch = kTile + kTile / 16;  // one pad word per 16
 
 __device__ __forceinline__ int tile_pad(int x) { return x + (x >> 4); }
 
+// Ordering for an LDS exchange among threads of one wave64 (a tile-pass
+// tile's threads): LDS operations of one wave complete in order, so a
+// wave-level barrier that keeps the compiler from moving memory operations
+// across it replaces a block barrier over all the block's waves.
+__device__ __forceinline__ void tile_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Inverse output representations (fh_ntt_inverse).
 // kOutStdMac: standard canonical output of accumulators whose binary64 rows
 // hold x*R mod q (a key MAC that skipped REDC's R^{-1}); integer rows are
