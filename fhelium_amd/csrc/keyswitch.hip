@@ -54,7 +54,6 @@ namespace {
 using namespace ::fh::ntt;
 
 constexpr int kMaxDigitRows = 8;
-constexpr int kMaxP = 8;
 constexpr int kModDownRowsPerThread = 8;
 
 // One active hybrid digit (RnsDigitSpec plus device table addresses),
@@ -348,6 +347,9 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
   const int col = blockIdx.x * C + c;
+  // 32-bit element offsets from the block-uniform row bases: the loads then
+  // take a scalar base plus a 32-bit vector offset (no 64-bit address adds)
+  auto at = [&](int k) { return static_cast<uint32_t>((t + T * k) * kTile + col); };
   V v[16];
   if constexpr (std::is_same<V, double>::value) {
     if (d.f64_ok) {
@@ -359,7 +361,7 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
       const double c0 = static_cast<double>(row.to_mont);
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
-        const double x = __longlong_as_double(dig0[static_cast<int64_t>(t + T * k) * kTile + col]);
+        const double x = __longlong_as_double(dig0[at(k)]);
         v[k] = std::is_same<P, F64Fast>::value ? f64_mulmod_signed(c0, x, q, qinv) : f64_mulmod(c0, x, q, qinv);
       }
 #pragma unroll
@@ -369,7 +371,7 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
         const int64_t* src = dig0 + r * n;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-          const double x = __longlong_as_double(src[static_cast<int64_t>(t + T * k) * kTile + col]);
+          const double x = __longlong_as_double(src[at(k)]);
           if constexpr (std::is_same<P, F64Fast>::value) {
             v[k] += f64_mulmod_signed(cr, x, q, qinv);
           } else {
@@ -395,7 +397,7 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
     auto term = [&](uint64_t w, uint64_t wp, uint64_t x) { return w * x + __umul64hi(wp, x) * neg_q; };
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const int64_t pos = static_cast<int64_t>(t + T * k) * kTile + col;
+      const uint32_t pos = at(k);
       v[k] = term(row.to_mont, row.to_mont_p, digit_u64(d, dig0[pos]));
     }
 #pragma unroll
@@ -407,7 +409,7 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
       const int64_t* src = dig0 + r * n;
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
-        const int64_t pos = static_cast<int64_t>(t + T * k) * kTile + col;
+        const uint32_t pos = at(k);
         v[k] += term(cr, crp, digit_u64(d, src[pos]));
       }
     }
@@ -465,7 +467,7 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
         hi[cc >> 1] = static_cast<uint32_t>(u >> 32) & 0xffffu;
       }
     }
-    const int64_t idx = j * n + packed_index(i, static_cast<int64_t>(blockIdx.x) * C + c16);
+    const uint32_t idx = static_cast<uint32_t>(j * n + packed_index(i, static_cast<int64_t>(blockIdx.x) * C + c16));
     uint4* dlo = reinterpret_cast<uint4*>(pk.lo + idx);
     uint4* dhi = reinterpret_cast<uint4*>(pk.hi + idx);
 #pragma unroll
@@ -474,7 +476,7 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
     dhi[1] = make_uint4(hi[4], hi[5], hi[6], hi[7]);
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) out[static_cast<int64_t>(16 * t + k) * kTile + col] = p.mid(v[k]);
+    for (int k = 0; k < 16; ++k) out[static_cast<uint32_t>((16 * t + k) * kTile + col)] = p.mid(v[k]);
   }
 }
 
@@ -602,7 +604,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       // in the packed block layout (one 32-byte and one 16-byte load); an
       // LDS pass moves them to the round-1 ownership.
       const PackedInter pk = packed_slab(a.inter + (b * a.dnum + di) * a.lqp * n, a.small_rows, n);
-      const int64_t idx = j * n + packed_index(r, 8 * t);
+      const uint32_t idx = static_cast<uint32_t>(j * n + packed_index(r, 8 * t));
       const uint4 l0 = *reinterpret_cast<const uint4*>(pk.lo + idx);
       const uint4 l1 = *reinterpret_cast<const uint4*>(pk.lo + idx + 4);
       const uint4 hh = *reinterpret_cast<const uint4*>(pk.hi + idx);
@@ -1125,11 +1127,11 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   const int64_t c = cb / batch;
   const int64_t b = cb % batch;
   int64_t* slab = const_cast<int64_t*>(a.acc) + (c * a.batch + b) * a.lqp * n;
-  const int64_t* x = slab + i;
   // Q rows below small_rows are packed binary64 rows (canonical residues)
   const PackedInter pk = packed_slab(slab, a.small_rows, n);
   auto acc_row = [&](int64_t row) -> int64_t {
-    return row < a.small_rows ? static_cast<int64_t>(packed_load(pk, row * n + pidx)) : x[row * n];
+    return row < a.small_rows ? static_cast<int64_t>(packed_load(pk, static_cast<uint32_t>(row * n + pidx)))
+                              : slab[static_cast<uint32_t>(row * n + i)];
   };
   const View3& base = c == 0 ? a.base0 : a.base1;
   const bool has_base = (c == 0 ? a.has_base0 : a.has_base1) != 0;
@@ -1167,7 +1169,9 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   // 2^30 M_r, P^{-1}, ..., q, 1/q): (x - X_P) P^{-1} mod q with
   // X_P mod q = sum_r c_lo M_r + c_hi 2^30 M_r, all in signed binary64
   // (|sum| < 4.2q), canonicalised once.
-  auto moddown_f64 = [&](Global<double>* rec, int64_t xval) -> int64_t {
+  // The result of a binary64 row stays in binary64 (exact integers below
+  // 2^47) through the base addition and the fused rescale.
+  auto moddown_f64 = [&](Global<double>* rec, int64_t xval) -> double {
     const double q = rec[14], qinv = rec[15];
     double xp = 0.0;
 #pragma unroll
@@ -1177,20 +1181,26 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     }
     const double diff = u64_to_f64(static_cast<uint64_t>(xval)) - xp;
     double res = f64_mulmod_signed(rec[2 * P], diff, q, qinv);
-    res = res < 0.0 ? res + q : res;
-    return static_cast<int64_t>(f64_to_u64(res));
+    return res < 0.0 ? res + q : res;
+  };
+  // add_canonical of a canonical value and a base below 2q
+  auto add_base_f64 = [&](double v, int64_t bval, double q) -> double {
+    if (!has_base) return v;
+    double sum = v + static_cast<double>(bval);
+    sum = sum >= 2.0 * q ? sum - 2.0 * q : sum;
+    return sum >= q ? sum - q : sum;
   };
   // Fused rescale: every thread needs the final row-0 value of its
   // coefficient (row 0 is a binary64 row: the dropped prime is a scale prime).
   const bool rescale = a.rescale_inv != nullptr;
-  int64_t c0 = 0;
-  int64_t round_up = 0;
+  double c0 = 0.0;
+  double round_up = 0.0;
   if (rescale) {
     Global<double>* rec0 = global(a.pdig.f64_table) + a.level * pitch;
     const int64_t x0 = row_begin == 0 ? xv[0] : acc_row(0);
     const int64_t b0 = row_begin == 0 ? bv[0] : (has_base ? base.at(b, 0, i) : 0);
-    c0 = with_base(b0, moddown_f64(rec0, x0), 2 * static_cast<int64_t>(static_cast<double>(rec0[14])));
-    round_up = c0 > a.half_drop ? 1 : 0;
+    c0 = add_base_f64(moddown_f64(rec0, x0), b0, rec0[14]);
+    round_up = c0 > static_cast<double>(a.half_drop) ? 1.0 : 0.0;
   }
 #pragma unroll
   for (int rr = 0; rr < kRows; ++rr) {
@@ -1201,16 +1211,14 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
     const double q = rec[14];
     int64_t value;
     if (q != 0.0) {
-      const int64_t twice_q = 2 * static_cast<int64_t>(q);
-      value = with_base(bv[rr], moddown_f64(rec, xv[rr]), twice_q);
+      double v = add_base_f64(moddown_f64(rec, xv[rr]), bv[rr], q);
       if (rescale && row > 0) {
-        // ((value - c0) q0^{-1} + round_up) mod q, |value - c0| < 2^42 exact
-        double r = f64_mulmod_signed(static_cast<double>(a.rescale_inv[row - 1]),
-                                     static_cast<double>(value - c0), q, rec[15]);
-        r += static_cast<double>(round_up);
-        r = r < 0.0 ? r + q : (r >= q ? r - q : r);
-        value = static_cast<int64_t>(f64_to_u64(r));
+        // ((v - c0) q0^{-1} + round_up) mod q, |v - c0| < 2^47 exact
+        double r = f64_mulmod_signed(static_cast<double>(a.rescale_inv[row - 1]), v - c0, q, rec[15]);
+        r += round_up;
+        v = r < 0.0 ? r + q : (r >= q ? r - q : r);
       }
+      value = static_cast<int64_t>(f64_to_u64(v));
     } else {
       // A prime above 2^46 (the base prime): the integer formula.
       const Mont m = load_mont(a.params, grow);
@@ -1225,7 +1233,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
       value = with_base(bv[rr], mont_mul(sub_lazy(xr, xp, m.twice_q), a.p_inverse[grow], m), m.twice_q);
       if (rescale && row > 0) {
         const uint64_t qq = m.q;
-        int64_t d = value - c0;
+        int64_t d = value - static_cast<int64_t>(c0);
         if (d < 0) d += static_cast<int64_t>(qq);
         uint64_t r = shoup_mulmod(static_cast<uint64_t>(a.rescale_inv[row - 1]),
                                   static_cast<uint64_t>(a.rescale_inv_shoup[row - 1]), static_cast<uint64_t>(d), qq);
