@@ -330,7 +330,7 @@ __device__ __forceinline__ uint64_t extend_integer(const KsDigit& d, const NttRo
 // ---------------------------------------------------------------------------
 template <int S1, int kRows, class P>
 __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const KsDigit& d,
-                                              int64_t di, int64_t b, int64_t j, int64_t* lds) {
+                                              int64_t di, int64_t b, int64_t j, int cg, int64_t* lds) {
   constexpr int R1 = 1 << S1;
   constexpr int T = R1 / 16;
   constexpr int C = kBlock / T;
@@ -346,7 +346,7 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
   constexpr int kMaxR = kRows == 0 ? kMaxDigitRows : kRows;
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int col = blockIdx.x * C + c;
+  const int col = cg * C + c;
   // 32-bit element offsets from the block-uniform row bases: the loads then
   // take a scalar base plus a 32-bit vector offset (no 64-bit address adds)
   auto at = [&](int k) { return static_cast<uint32_t>((t + T * k) * kTile + col); };
@@ -467,7 +467,7 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
         hi[cc >> 1] = static_cast<uint32_t>(u >> 32) & 0xffffu;
       }
     }
-    const uint32_t idx = static_cast<uint32_t>(j * n + packed_index(i, static_cast<int64_t>(blockIdx.x) * C + c16));
+    const uint32_t idx = static_cast<uint32_t>(j * n + packed_index(i, static_cast<int64_t>(cg) * C + c16));
     uint4* dlo = reinterpret_cast<uint4*>(pk.lo + idx);
     uint4* dhi = reinterpret_cast<uint4*>(pk.hi + idx);
 #pragma unroll
@@ -482,31 +482,39 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
 
 // One launch per arithmetic path, as for the tile pass below.
 template <int S1, int MODE, bool kSmall>
-__global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j_begin) {
+__global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j_begin, int64_t j_count) {
   // (+1 column of padding: the packed store's transpose pitch)
   __shared__ int64_t lds[(1 << S1) * (kBlock / ((1 << S1) / 16) + 1)];
-  const int64_t j = j_begin + blockIdx.y;
+  // Work items (column group fastest, then destination row, then (entry,
+  // digit)), XCD-contiguous: every block that reads one (entry, digit)'s
+  // source rows runs on one XCD, so those rows are fetched into one L2.
+  constexpr uint32_t kGroups = (1u << S1) / 16;
+  const uint32_t jc = static_cast<uint32_t>(j_count);
   const uint32_t dnum = static_cast<uint32_t>(a.dnum);
-  const int64_t b = blockIdx.z / dnum;
-  const int64_t di = blockIdx.z % dnum;
+  const uint32_t w = xcd_remap(blockIdx.x, kGroups * jc * dnum * static_cast<uint32_t>(a.batch));
+  const int cg = static_cast<int>(w % kGroups);
+  const int64_t j = j_begin + (w / kGroups) % jc;
+  const uint32_t z = w / (kGroups * jc);
+  const int64_t b = z / dnum;
+  const int64_t di = z % dnum;
   const KsDigit& d = a.dig[di];
   const NttRow& row = a.rows[a.level + j];
   if constexpr (kSmall && MODE == 1) {
     // the digit width is uniform over the block: a fixed-width body lets
     // every digit load issue before the extension arithmetic
     switch (static_cast<int>(d.rows)) {
-      case 1: modup_columns<S1, 1>(F64Fast(row), a, d, di, b, j, lds); break;
-      case 2: modup_columns<S1, 2>(F64Fast(row), a, d, di, b, j, lds); break;
-      case 4: modup_columns<S1, 4>(F64Fast(row), a, d, di, b, j, lds); break;
-      default: modup_columns<S1, 0>(F64Fast(row), a, d, di, b, j, lds); break;
+      case 1: modup_columns<S1, 1>(F64Fast(row), a, d, di, b, j, cg, lds); break;
+      case 2: modup_columns<S1, 2>(F64Fast(row), a, d, di, b, j, cg, lds); break;
+      case 4: modup_columns<S1, 4>(F64Fast(row), a, d, di, b, j, cg, lds); break;
+      default: modup_columns<S1, 0>(F64Fast(row), a, d, di, b, j, cg, lds); break;
     }
   } else if constexpr (kSmall) {
-    modup_columns<S1, 0>(F64Exact(row), a, d, di, b, j, lds);
+    modup_columns<S1, 0>(F64Exact(row), a, d, di, b, j, cg, lds);
   } else {
     switch (static_cast<int>(d.rows)) {
-      case 1: modup_columns<S1, 1>(U64Fast(row), a, d, di, b, j, lds); break;
-      case 2: modup_columns<S1, 2>(U64Fast(row), a, d, di, b, j, lds); break;
-      default: modup_columns<S1, 0>(U64Fast(row), a, d, di, b, j, lds); break;
+      case 1: modup_columns<S1, 1>(U64Fast(row), a, d, di, b, j, cg, lds); break;
+      case 2: modup_columns<S1, 2>(U64Fast(row), a, d, di, b, j, cg, lds); break;
+      default: modup_columns<S1, 0>(U64Fast(row), a, d, di, b, j, cg, lds); break;
     }
   }
 }
@@ -1371,15 +1379,17 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
     const int64_t j_begin = pass == 0 ? 0 : small_rows;
     const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
     if (j_count == 0) return kOk;
-    const dim3 gcol(kTile / C, static_cast<uint32_t>(j_count), static_cast<uint32_t>(digit_count * batch));
+    const int64_t modup_blocks = (kTile / C) * j_count * digit_count * batch;
+    FH_CHECK_ARG(modup_blocks <= 0x7fffffff, "key switch grid too large");
+    const dim3 gcol(static_cast<uint32_t>(modup_blocks));
 #define FH_MODUP_CASE(S)                                                                           \
   case S:                                                                                          \
     if (pass == 1) {                                                                               \
-      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, false>), gcol, dim3(kBlock), 0, st, a, j_begin); \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, false>), gcol, dim3(kBlock), 0, st, a, j_begin, j_count); \
     } else if (mode == 1) {                                                                        \
-      hipLaunchKernelGGL((k_ks_modup_columns<S, 1, true>), gcol, dim3(kBlock), 0, st, a, j_begin);  \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 1, true>), gcol, dim3(kBlock), 0, st, a, j_begin, j_count);  \
     } else {                                                                                       \
-      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, true>), gcol, dim3(kBlock), 0, st, a, j_begin);  \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, true>), gcol, dim3(kBlock), 0, st, a, j_begin, j_count);  \
     }                                                                                              \
     break;
     switch (s1) {
