@@ -54,7 +54,7 @@ namespace {
 using namespace ::fh::ntt;
 
 constexpr int kMaxDigitRows = 8;
-constexpr int kModDownRowsPerThread = 8;
+constexpr int kModDownRowsPerThread = 12;
 
 // One active hybrid digit (RnsDigitSpec plus device table addresses),
 // packed by the host as 16 int64 words.  Also used for the P primes.
