@@ -268,6 +268,18 @@ __device__ __forceinline__ uint64_t wide_redc64(const WideAcc& a, uint64_t q, ui
   return a.c2 + __umul64hi(m, q) + (lo != 0 ? 1u : 0u);
 }
 
+// 4 r reduced into [0, 2q), r = wide_redc64(sum) below 9q: the residue
+// sum 2^-62, i.e. what REDC with radix 2^62 (mont_mul) gives for a single
+// product, in the lazy range of the integer inverse butterflies.
+__device__ __forceinline__ uint64_t wide_redc62(const WideAcc& a, uint64_t q, uint64_t neg_qinv64) {
+  uint64_t r = wide_redc64(a, q, neg_qinv64);
+  r = csub(r, 8 * q);
+  r = csub(r, 4 * q);
+  r = csub(r, 2 * q);
+  r = csub(r, q);
+  return csub(r << 2, 2 * q);
+}
+
 // Exact u64 <-> f64 conversions for integers below 2^52 (every value the
 // binary64 paths convert is below 2^47): 2^52 + v has the bit pattern
 // 0x4330000000000000 | v, so each direction is one integer and one binary64

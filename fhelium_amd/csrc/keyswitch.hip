@@ -702,19 +702,10 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     // value r = sum 2^-64 mod q is below 128 q^2 / 2^64 + q < 9q (q < 2^60).
     // The binary64 rows' REDC radix is 2^62: 4 r is the same residue as
     // sum 2^-62.  Result in [0, 2q), the inverse butterflies' input range.
-    const uint64_t q = row.q;
-    auto finish = [&](const WideAcc& acc) {
-      uint64_t r = wide_redc64(acc, q, row.neg_qinv64);
-      r = csub(r, 8 * q);
-      r = csub(r, 4 * q);
-      r = csub(r, 2 * q);
-      r = csub(r, q);
-      return csub(r << 2, 2 * q);
-    };
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      uacc0[k] = finish(wacc0[k]);
-      uacc1[k] = finish(wacc1[k]);
+      uacc0[k] = wide_redc62(wacc0[k], row.q, row.neg_qinv64);
+      uacc1[k] = wide_redc62(wacc1[k], row.q, row.neg_qinv64);
     }
   }
   // The first 8 stages of the accumulators' inverse NTT act inside this
@@ -942,6 +933,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
   V d0[8], d1[8], d2[8];
   {
     V x0[8], x1[8], y[8];
+    WideAcc w1[kF64 ? 1 : 8];  // integer rows: d1's exact two-product sum
     forward_tile(0, x0);  // a0
     forward_tile(1, x1);  // a1
     forward_tile(2, y);   // b0
@@ -955,15 +947,15 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
         d1[k] = f64_mulmod_signed(x1[k], yb, row.qd, row.qinv);
       }
     } else {
-      // forward outputs lie in [0, 4q): halve to [0, 2q) for the REDC bound
-      const Mont mont = load_mont(a.params, grow);
+      // exact products of the forward outputs (below 4q), one REDC per
+      // result (sums below 32 q^2: REDC value below 3q)
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        x0[k] = p.half(x0[k]);
-        x1[k] = p.half(x1[k]);
-        y[k] = p.half(y[k]);
-        d0[k] = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(x0[k]), static_cast<int64_t>(y[k]), mont));
-        d1[k] = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(x1[k]), static_cast<int64_t>(y[k]), mont));
+        WideAcc w0{0, 0, 0};
+        wide_mac(w0, x0[k], y[k]);
+        d0[k] = wide_redc62(w0, row.q, row.neg_qinv64);
+        w1[k] = WideAcc{0, 0, 0};
+        wide_mac(w1[k], x1[k], y[k]);
       }
     }
     forward_tile(3, y);   // b1
@@ -975,14 +967,13 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
         d2[k] = f64_mulmod_signed(x1[k], yb, row.qd, row.qinv);
       }
     } else {
-      const Mont mont = load_mont(a.params, grow);
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        y[k] = p.half(y[k]);
-        d1[k] = static_cast<uint64_t>(add_lazy(
-            static_cast<int64_t>(d1[k]), mont_mul(static_cast<int64_t>(x0[k]), static_cast<int64_t>(y[k]), mont),
-            mont.twice_q));
-        d2[k] = static_cast<uint64_t>(mont_mul(static_cast<int64_t>(x1[k]), static_cast<int64_t>(y[k]), mont));
+        wide_mac(w1[k], x0[k], y[k]);
+        d1[k] = wide_redc62(w1[k], row.q, row.neg_qinv64);
+        WideAcc w2{0, 0, 0};
+        wide_mac(w2, x1[k], y[k]);
+        d2[k] = wide_redc62(w2, row.q, row.neg_qinv64);
       }
     }
   }
