@@ -224,8 +224,9 @@ int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lq
  * coefficient domain with canonical standard residues -- the composition
  * coefficient_domain_to_ntt_domain (ckks_engine.py:876-903) of both inputs,
  * multiply (:1530-1606) and the inverse NTT that starts relinearize
- * (:1608-1640), bit for bit, in six launches (four forward column passes,
- * one fused tile pass per arithmetic path, one inverse column pass).
+ * (:1608-1640), bit for bit, in five launches (one forward column pass per
+ * operand -- two when its components are not one [2 batch] run -- one fused
+ * tile pass per arithmetic path, one inverse column pass).
  *   a, b             [2, batch, limbs, N] canonical standard rows at level
  *                    row_start (component, batch, limb strides given;
  *                    coefficient stride 1)
