@@ -534,15 +534,18 @@ __device__ __forceinline__ void mac_f64(double& acc, double y, int64_t key, doub
 
 
 // Tile pass with 8 elements per thread (32 threads per 256-element tile,
-// 8 tiles per block): rounds of 3, 3 and 2 stages exchanged through LDS.
+// 4 tiles per 128-thread block): rounds of 3, 3 and 2 stages exchanged
+// through LDS.
 // Each tile's 255 twiddles (they do not depend on the digit) and the key
 // digit indices are staged in LDS once per block, so the digit loop issues
 // only the tile and key loads -- together, at the top of the iteration --
 // and the per-thread state is small enough for 4 waves per SIMD.
 constexpr int kMacThreads = 32;                     // threads per tile
-constexpr int kMacTiles = kBlock / kMacThreads;     // tiles per block
+constexpr int kMacBlock = 128;                      // threads per tile-pass block
+constexpr int kMacTiles = kMacBlock / kMacThreads;  // tiles per block
 constexpr int kMacExchange = kMacTiles * kTilePitch;
 constexpr int kMacMaxDigits = 16;
+static_assert(kMacMaxDigits <= kMacBlock, "one thread per key digit index");
 
 // The 32 threads of one tile are half of one wave64 and each tile's LDS
 // exchange buffer and twiddles are its own, so in-tile exchanges use
@@ -807,7 +810,7 @@ constexpr int kMacTilesPerBlock = kMacTiles;
 template <int MODE, bool kSmall>
 // Register budgets: 4 waves per SIMD for the binary64 rows (a few spilled
 // dwords cost less than the lost occupancy), 2+ for the integer rows.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
+__global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
                                                          int64_t j_count) {
   __shared__ int64_t lds[mac_lds_words<kSmall>()];
   const uint32_t batch = static_cast<uint32_t>(a.batch);
@@ -1044,7 +1047,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
 }
 
 template <bool kSmall>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_hmult_tiles(
+__global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_hmult_tiles(
     HmArgs a, int64_t groups, int64_t j_begin, int64_t j_count) {
   __shared__ int64_t lds[kMacExchange + kMacTiles * kTile * (kSmall ? 1 : 2)];
   const uint32_t batch = static_cast<uint32_t>(a.batch);
@@ -1401,11 +1404,11 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
     if (j_count == 0) return kOk;
     const dim3 grid(static_cast<uint32_t>(groups * j_count * batch));
     if (pass == 1) {
-      hipLaunchKernelGGL((k_ks_tiles_mac<0, false>), grid, dim3(kBlock), 0, st, a, groups, j_begin, j_count);
+      hipLaunchKernelGGL((k_ks_tiles_mac<0, false>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);
     } else if (mode == 1) {
-      hipLaunchKernelGGL((k_ks_tiles_mac<1, true>), grid, dim3(kBlock), 0, st, a, groups, j_begin, j_count);
+      hipLaunchKernelGGL((k_ks_tiles_mac<1, true>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);
     } else {
-      hipLaunchKernelGGL((k_ks_tiles_mac<0, true>), grid, dim3(kBlock), 0, st, a, groups, j_begin, j_count);
+      hipLaunchKernelGGL((k_ks_tiles_mac<0, true>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);
     }
     FH_CHECK_LAUNCH("key switch tile pass + MAC");
     return kOk;
@@ -1563,12 +1566,12 @@ FH_API int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, 
   FH_CHECK_ARG(groups * limbs * batch <= 0x7fffffff, "fused tensor product grid too large");
   if (small_rows > 0) {
     hipLaunchKernelGGL(k_hmult_tiles<true>, dim3(static_cast<uint32_t>(groups * small_rows * batch)),
-                       dim3(kBlock), 0, s, h, groups, int64_t{0}, small_rows);
+                       dim3(kMacBlock), 0, s, h, groups, int64_t{0}, small_rows);
     FH_CHECK_LAUNCH("fused tensor product tiles (binary64 rows)");
   }
   if (small_rows < limbs) {
     hipLaunchKernelGGL(k_hmult_tiles<false>, dim3(static_cast<uint32_t>(groups * (limbs - small_rows) * batch)),
-                       dim3(kBlock), 0, s, h, groups, small_rows, limbs - small_rows);
+                       dim3(kMacBlock), 0, s, h, groups, small_rows, limbs - small_rows);
     FH_CHECK_LAUNCH("fused tensor product tiles (integer rows)");
   }
   return ntt::inverse_columns_internal(out, 3 * batch, limbs, static_cast<int>(log_n), limbs * n, n, rows,

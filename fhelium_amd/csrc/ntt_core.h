@@ -261,9 +261,9 @@ struct U64Fast {
 // plane follows it; both fit inside the slab's first small_rows rows, so the
 // integer rows keep their int64 layout after them.  Within a row the
 // coefficients are stored in 8 x 16 blocks of the 2^(logN-8) x 256 matrix
-// (row i = tile, column c): a band of 8 rows is the 8 tiles one tile-pass
-// block covers, and one 16-column block row is what a column-pass wave
-// covers, so both passes move whole cache lines.
+// (row i = tile, column c): a tile-pass thread reads or writes 8 or 16
+// consecutive coefficients of a 16-column run, and one 16-column block row
+// is what a column-pass wave covers, so both passes move whole cache lines.
 struct PackedInter {
   uint32_t* lo;
   uint16_t* hi;
