@@ -22,6 +22,9 @@ from fhelium_amd.runtime import RnsRuntime
 
 # Primes below this bound use the binary64 kernels (matches the NTT row records).
 SMALL_PRIME_LIMIT = 1 << 46
+# The integer rows' lazy arithmetic (forward values below 4q, exact product
+# sums with one REDC) is sized for primes below this.
+INTEGER_PRIME_LIMIT = 1 << 60
 
 
 class HybridKeySwitcher:
@@ -29,6 +32,9 @@ class HybridKeySwitcher:
         self.rt = runtime
         self.layout = runtime.layout
         self.config = runtime.config
+        too_large = [q for q in self.config.moduli if q >= INTEGER_PRIME_LIMIT]
+        if too_large:
+            raise ValueError(f"fused key switch needs every prime below 2^60, got {too_large}")
         self._digit_tables: dict[int, torch.Tensor] = {}
         # Optional list receiving (start_event, end_event, algorithmic_bytes)
         # per fused key switch, recorded on the operand's stream.

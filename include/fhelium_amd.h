@@ -178,7 +178,8 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
  * replaces the coefficient automorphism of a rotation (ckks_engine.py:3822-3897).
  *   component        [batch, lq, N] canonical standard Q rows at `level`
  *   small_rows       number of leading QP rows at `level` whose prime is
- *                    below 2^46 (binary64 arithmetic; the rest are integer)
+ *                    below 2^46 (binary64 arithmetic; the rest are integer
+ *                    rows, whose primes must be below 2^60)
  *   digit_table      [digit_count, 16] int64 digit descriptors (row_begin,
  *                    rows, key_digit, addresses of the digit's mixed-radix
  *                    and basis-extension tables, basis row stride, address
