@@ -17,7 +17,7 @@ import threading
 import torch
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "_lib" / "libfhelium_amd.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class NativeExtensionError(RuntimeError):
@@ -80,7 +80,8 @@ SIGNATURES: dict[str, list] = {
                                                fh_table2, _i64, _i64, _i64, _i64, _ptr],
     "fh_keyswitch_fused": [fh_view3, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, _i64, _ptr, _ptr, fh_table2, _ptr,
                            _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr,
-                           _ptr, fh_view3, fh_view3, fh_view3, fh_view3, _i32, _ptr, _ptr, _i64, _ptr],
+                           _ptr, fh_view3, fh_view3, fh_view3, fh_view3, _i32, _ptr, _ptr, _i64, _ptr, _ptr,
+                           _ptr, _ptr],
     "fh_chacha20_keystream": [_ptr, _ptr, ctypes.c_uint32, _i64, _ptr, _ptr],
     "fh_sample_uniform_mod": [_ptr, _ptr, ctypes.c_uint32, _ptr, _i64, _i64, _i64, _ptr, _ptr],
     "fh_sample_ternary": [_ptr, _ptr, ctypes.c_uint32, _i64, _ptr, _ptr],
@@ -89,7 +90,7 @@ SIGNATURES: dict[str, list] = {
     "fh_ckks_decode": [_ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, ctypes.c_double, _ptr, _ptr, _ptr],
     "fh_ckks_hmult_tensor_coefficient": [_ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
                                          _i64, _i64, fh_table2, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr,
-                                         _ptr],
+                                         _i32, _ptr],
 }
 
 _lock = threading.Lock()

@@ -105,6 +105,11 @@ struct KsArgs {
   const int64_t* gal_idx;   // per-entry int32 gather table addresses [B] or null
   const int64_t* gal_sign;  // per-entry int8 sign table addresses [B]
   int64_t small_rows;       // leading QP rows on the binary64 path
+  // Optional accumulator seeds (null: none): [B, Lq, N] NTT-domain rows in
+  // the accumulators' scaling, added to the Q rows times seed_scale[row].
+  const int64_t* seed0;
+  const int64_t* seed1;
+  const int64_t* seed_scale;  // [Q0] P mod q (binary64 rows), P R mod q (integer rows)
 };
 
 // Block i runs on XCD i % 8; give each XCD one contiguous range of work
@@ -699,6 +704,28 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       }
     }
   }
+  // Relinearization seeds: P d0 and P d1 (NTT domain, in the accumulators'
+  // scaling) join the Q rows' sums, so that ModDown's exact floor division
+  // returns floor(X / P) + d directly.  Thread t holds coefficients 8t + k,
+  // stored at tile positions t + 32k (the HMult front's seed layout).
+  if (a.seed0 != nullptr && j < a.lq) {
+    const int64_t off = (b * a.lq + j) * n + tile_off + t;
+    if constexpr (kF64) {
+      const double sc = static_cast<double>(a.seed_scale[grow]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        facc0[k] += f64_mulmod_signed(sc, __longlong_as_double(a.seed0[off + 32 * k]), row.qd, row.qinv);
+        facc1[k] += f64_mulmod_signed(sc, __longlong_as_double(a.seed1[off + 32 * k]), row.qd, row.qinv);
+      }
+    } else {
+      const uint64_t sc = static_cast<uint64_t>(a.seed_scale[grow]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        wide_mac(wacc0[k], static_cast<uint64_t>(a.seed0[off + 32 * k]), sc);
+        wide_mac(wacc1[k], static_cast<uint64_t>(a.seed1[off + 32 * k]), sc);
+      }
+    }
+  }
   if constexpr (!kF64) {
     // Each sum holds at most 16 products y w with y < 4q (forward outputs)
     // and w < 2q (key words), so it is below 128 q^2 < 2^127 and its REDC
@@ -855,7 +882,9 @@ struct HmArgs {
   const int64_t* inv_wp;
 };
 
-template <class P>
+// kNttD01: d0, d1 stay in the NTT domain (a compile-time choice: a runtime
+// branch between the two tails costs the kernel its register allocation).
+template <bool kNttD01, class P>
 __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t g, int64_t j, int64_t b,
                                             int64_t* lds) {
   using V = typename P::V;
@@ -1041,12 +1070,24 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
     tile_sync();  // the exchange buffer is reused by the next product
   };
   int64_t* out = a.out + (b * a.limbs + j) * n + tile_off;
-  inverse_tile(d0, out);
-  inverse_tile(d1, out + comp_stride);
+  if constexpr (kNttD01) {
+    // d0, d1 stay in the NTT domain as the key switch's accumulator seeds.
+    // Seed layout: within each 256-coefficient tile, coefficient 8t + k (the
+    // ownership both this kernel and the MAC end in) sits at position
+    // t + 32k, so the stores and the MAC's loads are coalesced.
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      out[t + 32 * k] = p.mid(d0[k]);
+      out[comp_stride + t + 32 * k] = p.mid(d1[k]);
+    }
+  } else {
+    inverse_tile(d0, out);
+    inverse_tile(d1, out + comp_stride);
+  }
   inverse_tile(d2, out + 2 * comp_stride);
 }
 
-template <bool kSmall>
+template <bool kSmall, bool kNttD01>
 __global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_hmult_tiles(
     HmArgs a, int64_t groups, int64_t j_begin, int64_t j_count) {
   __shared__ int64_t lds[kMacExchange + kMacTiles * kTile * (kSmall ? 1 : 2)];
@@ -1058,9 +1099,9 @@ __global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmal
   const int64_t g = w / (batch * jc);
   const NttRow& row = a.rows[a.row_start + j];
   if constexpr (kSmall) {
-    tiles_hmult(F64Fast(row), a, g, j, b, lds);
+    tiles_hmult<kNttD01>(F64Fast(row), a, g, j, b, lds);
   } else {
-    tiles_hmult(U64Fast(row), a, g, j, b, lds);
+    tiles_hmult<kNttD01>(U64Fast(row), a, g, j, b, lds);
   }
 }
 
@@ -1311,7 +1352,9 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
                               int64_t* workspace_digits, int64_t* workspace_inter,
                               int64_t* workspace_acc, fh_view3 base0, fh_view3 base1, fh_view3 out0,
                               fh_view3 out1, int32_t mode, const int64_t* rescale_inverse,
-                              const int64_t* rescale_inverse_shoup, int64_t half_drop, void* stream) {
+                              const int64_t* rescale_inverse_shoup, int64_t half_drop,
+                              const int64_t* seed0, const int64_t* seed1, const int64_t* seed_scale,
+                              void* stream) {
   FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "fused key switch supports logN 12..16");
   FH_CHECK_ARG(batch >= 1 && batch <= 32767, "fused key switch batch %lld out of range",
                (long long)batch);
@@ -1326,6 +1369,8 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   FH_CHECK_ARG((key != nullptr) != (key_bases != nullptr), "pass exactly one of key / key_bases");
   FH_CHECK_ARG((galois_index_tables == nullptr) == (galois_sign_tables == nullptr),
                "Galois index and sign tables go together");
+  FH_CHECK_ARG((seed0 == nullptr) == (seed1 == nullptr) && (seed0 == nullptr || seed_scale != nullptr),
+               "accumulator seeds come in pairs with their scale table");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int64_t n = int64_t{1} << log_n;
   KsArgs a;
@@ -1355,6 +1400,9 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   a.gal_idx = galois_index_tables;
   a.gal_sign = galois_sign_tables;
   a.small_rows = small_rows;
+  a.seed0 = seed0;
+  a.seed1 = seed1;
+  a.seed_scale = seed_scale;
 
   FH_CHECK_ARG(small_rows >= 0 && small_rows <= lqp, "bad binary64 row count");
   const int s1 = static_cast<int>(log_n) - kTileLog;
@@ -1516,8 +1564,9 @@ FH_API int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, 
                                             const int64_t* forward_twiddles_shoup,
                                             const int64_t* inverse_twiddles,
                                             const int64_t* inverse_twiddles_shoup, int64_t* workspace,
-                                            int64_t* out, void* stream) {
+                                            int64_t* out, int32_t ntt_d01, void* stream) {
   FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "fused tensor product supports logN 12..16");
+  FH_CHECK_ARG(ntt_d01 == 0 || ntt_d01 == 1, "ntt_d01 is 0 or 1");
   FH_CHECK_ARG(batch >= 1 && limbs >= 1 && row_start >= 0, "bad batch, limb count or row start");
   FH_CHECK_ARG(small_rows >= 0 && small_rows <= limbs, "bad binary64 row count");
   FH_CHECK_ARG(a && b && rns_params.data && ntt_rows && forward_twiddles && forward_twiddles_shoup &&
@@ -1565,15 +1614,28 @@ FH_API int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, 
   const int64_t groups = n / (kTile * kMacTilesPerBlock);
   FH_CHECK_ARG(groups * limbs * batch <= 0x7fffffff, "fused tensor product grid too large");
   if (small_rows > 0) {
-    hipLaunchKernelGGL(k_hmult_tiles<true>, dim3(static_cast<uint32_t>(groups * small_rows * batch)),
-                       dim3(kMacBlock), 0, s, h, groups, int64_t{0}, small_rows);
+    const dim3 grid(static_cast<uint32_t>(groups * small_rows * batch));
+    if (ntt_d01) {
+      hipLaunchKernelGGL((k_hmult_tiles<true, true>), grid, dim3(kMacBlock), 0, s, h, groups, int64_t{0}, small_rows);
+    } else {
+      hipLaunchKernelGGL((k_hmult_tiles<true, false>), grid, dim3(kMacBlock), 0, s, h, groups, int64_t{0}, small_rows);
+    }
     FH_CHECK_LAUNCH("fused tensor product tiles (binary64 rows)");
   }
   if (small_rows < limbs) {
-    hipLaunchKernelGGL(k_hmult_tiles<false>, dim3(static_cast<uint32_t>(groups * (limbs - small_rows) * batch)),
-                       dim3(kMacBlock), 0, s, h, groups, small_rows, limbs - small_rows);
+    const dim3 grid(static_cast<uint32_t>(groups * (limbs - small_rows) * batch));
+    if (ntt_d01) {
+      hipLaunchKernelGGL((k_hmult_tiles<false, true>), grid, dim3(kMacBlock), 0, s, h, groups, small_rows,
+                         limbs - small_rows);
+    } else {
+      hipLaunchKernelGGL((k_hmult_tiles<false, false>), grid, dim3(kMacBlock), 0, s, h, groups, small_rows,
+                         limbs - small_rows);
+    }
     FH_CHECK_LAUNCH("fused tensor product tiles (integer rows)");
   }
-  return ntt::inverse_columns_internal(out, 3 * batch, limbs, static_cast<int>(log_n), limbs * n, n, rows,
-                                       inverse_twiddles, inverse_twiddles_shoup, row_start, kOutStdMac, 0, s);
+  // d2 (and d0, d1 unless they stay in the NTT domain) back to coefficients
+  const int64_t inv_comps = ntt_d01 ? 1 : 3;
+  return ntt::inverse_columns_internal(out + (3 - inv_comps) * comp, inv_comps * batch, limbs,
+                                       static_cast<int>(log_n), limbs * n, n, rows, inverse_twiddles,
+                                       inverse_twiddles_shoup, row_start, kOutStdMac, 0, s);
 }

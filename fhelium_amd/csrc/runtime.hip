@@ -23,7 +23,7 @@ const char* last_error() { return g_error; }
 
 FH_API const char* fh_last_error(void) { return fh::last_error(); }
 
-FH_API int fh_abi_version(void) { return 3; }
+FH_API int fh_abi_version(void) { return 4; }
 
 // Writes "<arch> cu=<n> lds=<bytes> l2=<bytes> mem=<bytes>" for the current
 // device into buf.

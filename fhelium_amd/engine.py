@@ -613,9 +613,10 @@ class CkksEngine:
         next_level = a.level + 1
         if next_level >= self.config.public_level_count:
             raise MaximumLevelError(a.level, self.config.public_level_count)
-        d = self.key_switcher.tensor_product_coefficient(a.data, b.data, a.level)
-        # relinearization and rescale in one key switch (ModDown writes the
-        # rescaled rows)
-        out = self.key_switcher.key_switch_fused(d[2], key, a.level, base0=d[0], base1=d[1], rescale=True)
+        # d0, d1 stay in the NTT domain and seed the key switch's accumulators
+        # (the result gains exactly d0, d1); relinearization and rescale run
+        # in one key switch (ModDown writes the rescaled rows)
+        d = self.key_switcher.tensor_product_coefficient(a.data, b.data, a.level, ntt_d01=True)
+        out = self.key_switcher.key_switch_fused(d[2], key, a.level, rescale=True, seeds=d[:2])
         dropped = self.config.moduli[self.runtime.row_start(a.level)]
         return Ciphertext(out, next_level, scale / float(dropped), "coefficient", "standard", "Q")

@@ -112,12 +112,18 @@ class HybridKeySwitcher:
     def key_switch_fused(self, component: torch.Tensor, key: KeySwitchKey | list[KeySwitchKey],
                          level: int, base0: torch.Tensor | None = None,
                          base1: torch.Tensor | None = None,
-                         galois_elements: list[int] | None = None, rescale: bool = False) -> torch.Tensor:
+                         galois_elements: list[int] | None = None, rescale: bool = False,
+                         seeds: torch.Tensor | None = None) -> torch.Tensor:
         """Whole key switch in eight launches; returns [2, *batch, Q_level, N]
         holding (base0 + corr0, base1 + corr1) canonically added, or the bare
         corrections where a base is None.  With ``rescale`` the result is
         also rescaled in the same ModDown launch (rescale_to_next_level's
         nearest rounding) and has Q_level - 1 rows.
+
+        ``seeds`` ([2, *batch, Q_level, N], the NTT-domain d0, d1 that
+        ``tensor_product_coefficient(..., ntt_d01=True)`` leaves) is added to
+        the results exactly as base0 = d0, base1 = d1 would be, through the
+        accumulators (floor((X + P d) / P) = floor(X / P) + d).
 
         With ``galois_elements`` (one per batch entry) the component and base0
         are read through entry b's coefficient automorphism, i.e. this is the
@@ -165,6 +171,12 @@ class HybridKeySwitcher:
             tables = [rt.coefficient_galois(int(g)) for g in galois_elements]
             gal_idx_ptr = self._address_table([t[0] for t in tables], dev).data_ptr()
             gal_sgn_ptr = self._address_table([t[1] for t in tables], dev).data_ptr()
+        seed_ptrs = (None, None, None)
+        if seeds is not None:
+            sv = seeds.reshape(2, -1, lq, n)
+            if sv.size(1) != b or not sv.is_contiguous():
+                raise ValueError("seeds must be a contiguous [2, *batch, Q_level, N] tensor")
+            seed_ptrs = (sv[0].data_ptr(), sv[1].data_ptr(), rt.seed_scale.data_ptr())
         none = nat.fh_view3(None, 0, 0, 0)
         view = lambda t: nat.v3(t.reshape(-1, lq, n)) if t is not None else none  # noqa: E731
         nt = rt.ntt_tables
@@ -185,11 +197,12 @@ class HybridKeySwitcher:
                  ref.stride(0), ref.stride(1), ref.stride(2), gal_idx_ptr, gal_sgn_ptr,
                  ws_digits.data_ptr(), ws_inter.data_ptr(), ws_acc.data_ptr(), view(base0), view(base1),
                  nat.v3(out[0]), nat.v3(out[1]), rt.ntt._mode, *self._rescale_args(level, rescale),
-                 nat.stream_ptr(dev))
+                 *seed_ptrs, nat.stream_ptr(dev))
         if rec is not None:
             ev1.record()
+            addends = (base0 is not None) + (base1 is not None) + (2 if seeds is not None else 0)
             rec.append((ev0, ev1, self.algorithmic_bytes(b, lq, lqp, dnum, isinstance(key, (list, tuple)),
-                                                         (base0 is not None) + (base1 is not None))))
+                                                         addends)))
         # Workspaces and the address tables may be released now: PyTorch's
         # caching allocator only hands their blocks to later work on this
         # same stream, which is ordered after these launches.
@@ -219,11 +232,14 @@ class HybridKeySwitcher:
             cache[level] = hit
         return hit[0].data_ptr(), hit[1].data_ptr(), hit[2]
 
-    def tensor_product_coefficient(self, a: torch.Tensor, b: torch.Tensor, level: int) -> torch.Tensor:
+    def tensor_product_coefficient(self, a: torch.Tensor, b: torch.Tensor, level: int,
+                                   ntt_d01: bool = False) -> torch.Tensor:
         """[3, *batch, Q_l, N] coefficient-domain d0, d1, d2 of two
         coefficient-domain ciphertexts a, b ([2, *batch, Q_l, N], canonical
         standard): NTT of both, tensor product and inverse NTT in one fused
-        pass (fh_ckks_hmult_tensor_coefficient, fast arithmetic)."""
+        pass (fh_ckks_hmult_tensor_coefficient, fast arithmetic).  With
+        ``ntt_d01`` only d2 returns to the coefficient domain; d0, d1 stay in
+        the NTT domain as ``key_switch_fused``'s ``seeds``."""
 
         from fhelium_amd import _native as nat
 
@@ -252,7 +268,7 @@ class HybridKeySwitcher:
                  av.stride(2), bv.stride(0), bv.stride(1), bv.stride(2), batch, lq, cfg.logN, start, small_rows,
                  nat.t2(rt.rns_params), nt.rows.data_ptr(), nt.fwd.data_ptr(), nt.fwd_shoup.data_ptr(),
                  nt.inv.data_ptr(), nt.inv_shoup.data_ptr(), ws.data_ptr(), out.data_ptr(),
-                 nat.stream_ptr(a.device))
+                 1 if ntt_d01 else 0, nat.stream_ptr(a.device))
         return out.reshape(3, *batch_shape, lq, n)
 
     def algorithmic_bytes(self, batch: int, lq: int, lqp: int, dnum: int, per_entry_keys: bool,

@@ -252,6 +252,13 @@ class HostTables:
         self.p_inverse_q = torch.tensor(
             [pow(p_product, -1, moduli[i]) for i in self.layout.prime_ids(0)], dtype=torch.int64
         )
+        # Key-switch accumulator seed scales: P mod q for the binary64 rows
+        # (their accumulators hold x R^2 and the seeds a b R^2), P R mod q for
+        # the integer rows (exact sums reduced by one REDC: x R, seeds a b R).
+        self.seed_scale = torch.tensor(
+            [p_product % moduli[i] if moduli[i] < (1 << 46) else (p_product * R) % moduli[i]
+             for i in self.layout.prime_ids(0)], dtype=torch.int64
+        )
         self.moddown_binary64 = moddown_binary64_table(moduli, self.layout.p_prime_ids,
                                                        self.layout.prime_ids(0))
 

@@ -68,6 +68,7 @@ class RnsRuntime:
         self.p_product_montgomery_q = self.host.p_product_montgomery_q.to(dev)
         self.p_digit = DeviceDigitTables.from_host(self.layout.p_prime_ids, self.host.p_digit, dev)
         self.p_inverse_q = self.host.p_inverse_q.to(dev)
+        self.seed_scale = self.host.seed_scale.to(dev)
         self.moddown_binary64 = self.host.moddown_binary64.to(dev).contiguous()
         self._galois_cache: dict[int, tuple[torch.Tensor, torch.Tensor]] = {}
         self._ntt_galois_cache: dict[int, torch.Tensor] = {}

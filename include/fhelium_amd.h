@@ -204,7 +204,15 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
  *                    half_drop = floor(q_level / 2): the results are then
  *                    rescaled as rescale_to_next_level does
  *                    (ckks_engine.py:1131-1174, ckks_cpu.cpp:204-275) and
- *                    out0 / out1 receive lq - 1 rows. */
+ *                    out0 / out1 receive lq - 1 rows.
+ *   seed0 / seed1    NULL, or [batch, lq, N] contiguous NTT-domain rows in
+ *                    the accumulators' scaling (what
+ *                    fh_ckks_hmult_tensor_coefficient leaves in d0, d1 with
+ *                    ntt_d01 = 1), with seed_scale [Q0] (P mod q for rows
+ *                    below 2^46, P 2^62 mod q for the others): seed_scale x
+ *                    seed is added to the Q rows of accumulator 0 / 1, so the
+ *                    results gain exactly d0 / d1 (floor((X + P d) / P) =
+ *                    floor(X / P) + d), as base0 / base1 = d0 / d1 would. */
 int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp, int64_t level,
                        int64_t log_n, int64_t small_rows, const int64_t* digit_table,
                        int64_t digit_count,
@@ -218,7 +226,8 @@ int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lq
                        int64_t* workspace_inter, int64_t* workspace_acc, fh_view3 base0,
                        fh_view3 base1, fh_view3 out0, fh_view3 out1, int32_t mode,
                        const int64_t* rescale_inverse, const int64_t* rescale_inverse_shoup,
-                       int64_t half_drop, void* stream);
+                       int64_t half_drop, const int64_t* seed0, const int64_t* seed1,
+                       const int64_t* seed_scale, void* stream);
 
 /* HMult front of the batching tier: d0 = a0 b0, d1 = a0 b1 + a1 b0,
  * d2 = a1 b1 of two coefficient-domain ciphertexts, returned in the
@@ -234,6 +243,13 @@ int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lq
  *   small_rows       leading rows whose prime is below 2^46
  *   workspace        [4, batch, limbs, N] contiguous
  *   out              [3, batch, limbs, N] contiguous
+ *   ntt_d01          0: d0, d1, d2 all in the coefficient domain; 1: d0 and
+ *                    d1 stay in the NTT domain as accumulator seeds for
+ *                    fh_keyswitch_fused (working format: binary64 bits of
+ *                    a b R^2 for rows below 2^46, a b R as u64 below 2q for
+ *                    the others; within each 256-coefficient tile the value
+ *                    of coefficient 8t + k at position t + 32k), d2 in the
+ *                    coefficient domain
  * Fast NTT arithmetic only (the engine's default mode). */
 int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, int64_t a_comp_stride,
                                      int64_t a_batch_stride, int64_t a_limb_stride, int64_t b_comp_stride,
@@ -242,7 +258,8 @@ int fh_ckks_hmult_tensor_coefficient(const int64_t* a, const int64_t* b, int64_t
                                      fh_table2 rns_params, const void* ntt_rows,
                                      const int64_t* forward_twiddles, const int64_t* forward_twiddles_shoup,
                                      const int64_t* inverse_twiddles, const int64_t* inverse_twiddles_shoup,
-                                     int64_t* workspace, int64_t* out, void* stream);
+                                     int64_t* workspace, int64_t* out, int32_t ntt_d01,
+                                     void* stream);
 
 /* ---- Sampling: this backend's own ChaCha20 generator ------------------
  * Replaces the reference's ChaCha20 streams (fhelium/rng/csprng.py:150-250,

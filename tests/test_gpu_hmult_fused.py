@@ -76,3 +76,22 @@ def test_keyswitch_with_fused_rescale(eng, level):
     bare = ks.key_switch_fused(ct.data[1], key, level)
     bare_r = ks.key_switch_fused(ct.data[1], key, level, rescale=True)
     assert torch.equal(bare_r, eng.rescale_to_next_level(Ciphertext(bare, level, ct.scale)).data)
+
+
+@pytest.mark.parametrize("level,rescale", [(0, True), (12, False), (30, True)])
+def test_seeded_keyswitch_matches_bases(eng, level, rescale):
+    """d0, d1 left in the NTT domain and added through the accumulators
+    (floor((X + P d) / P) = floor(X / P) + d) give the same key switch, bit
+    for bit, as adding the coefficient-domain d0, d1 after ModDown."""
+    rng = np.random.default_rng(100 + level)
+    x = rng.uniform(-1, 1, (3, eng.config.slots))
+    y = rng.uniform(-1, 1, (3, eng.config.slots))
+    a, b = eng.encrypt(x, level=level), eng.encrypt(y, level=level)
+    ks = eng.key_switcher
+    key = eng.relinearization_key
+    d = ks.tensor_product_coefficient(a.data, b.data, level)
+    seeded = ks.tensor_product_coefficient(a.data, b.data, level, ntt_d01=True)
+    assert torch.equal(seeded[2], d[2])
+    want = ks.key_switch_fused(d[2], key, level, base0=d[0], base1=d[1], rescale=rescale)
+    got = ks.key_switch_fused(seeded[2], key, level, rescale=rescale, seeds=seeded[:2])
+    assert torch.equal(got, want)
