@@ -1088,7 +1088,7 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
 }
 
 template <bool kSmall, bool kNttD01>
-__global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_hmult_tiles(
+__global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 3 : 2))) void k_hmult_tiles(
     HmArgs a, int64_t groups, int64_t j_begin, int64_t j_count) {
   __shared__ int64_t lds[kMacExchange + kMacTiles * kTile * (kSmall ? 1 : 2)];
   const uint32_t batch = static_cast<uint32_t>(a.batch);
