@@ -837,7 +837,7 @@ constexpr int kMacTilesPerBlock = kMacTiles;
 template <int MODE, bool kSmall>
 // Register budgets: 4 waves per SIMD for the binary64 rows (a few spilled
 // dwords cost less than the lost occupancy), 2+ for the integer rows.
-__global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 4 : 2))) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
+__global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 3 : 2))) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
                                                          int64_t j_count) {
   __shared__ int64_t lds[mac_lds_words<kSmall>()];
   const uint32_t batch = static_cast<uint32_t>(a.batch);
