@@ -40,10 +40,10 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) per algorithmic
 # byte of the fused key switch on this workload; the rocprofv3 --pmc passes
 # and the arithmetic are recorded in profiles/keyswitch_fused_traffic.md.
-KS_TRAFFIC_BYTES_PER_ALG_BYTE = 3.083
+KS_TRAFFIC_BYTES_PER_ALG_BYTE = 2.926
 # Measured VALU wave-instructions (SQ_INSTS_VALU) per algorithmic byte of the
 # fused key switch on this workload (profiles/keyswitch_fused_valu_issue.md).
-KS_VALU_INSTS_PER_ALG_BYTE = 0.33420
+KS_VALU_INSTS_PER_ALG_BYTE = 0.33357
 # gfx950 issues one wave64 VALU instruction per CU per cycle for full-rate
 # operations (the rate behind the 78.6 TFLOP/s binary64 FMA peak):
 # 256 CUs x 2.4 GHz.
