@@ -342,7 +342,7 @@ def main() -> int:
 
     log(nat.device_info())
     t_setup = time.perf_counter()
-    eng = CkksEngine(Preset.slots32768_scale40_levels34_int64, ntt_mode=args.ntt_mode, seed=args.seed)
+    eng = CkksEngine(Preset.slots32768_scale40_levels34_int64, ntt_mode=args.ntt_mode, rng_seed=args.seed)
     eng.generate_secret_key()
     eng.generate_relinearization_key()
     levels, steps, groups = build_workload(eng, args.count, args.seed)

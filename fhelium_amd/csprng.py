@@ -10,6 +10,11 @@ seed, read as eight little-endian words); the nonce is (low 32 bits, high 32
 bits, stream word), and the stream word advances when a request would wrap
 the 32-bit block counter.
 
+Without a seed the 256-bit key and the 64-bit nonce come from the operating
+system's entropy (``secrets``), as the reference leaves production entropy to
+its generator when ``rng_seed`` is None (fhelium/engine/ckks_engine.py:72-74);
+fixed seeds are for tests and benchmarks.
+
 The reference's own streams cannot be reproduced here (its package is not
 available), so keys and ciphertexts differ from the reference's for equal
 seeds; the block function itself is checked against RFC 8439.
@@ -19,6 +24,7 @@ from __future__ import annotations
 
 import ctypes
 import hashlib
+import secrets
 
 import torch
 
@@ -32,10 +38,20 @@ def key_words_from_seed(seed: int) -> list[int]:
     return [int.from_bytes(digest[i: i + 4], "little") for i in range(0, 32, 4)]
 
 
+def entropy_key_words() -> list[int]:
+    """Eight 32-bit key words from the operating system's CSPRNG."""
+
+    raw = secrets.token_bytes(32)
+    return [int.from_bytes(raw[i: i + 4], "little") for i in range(0, 32, 4)]
+
+
 class Csprng:
-    def __init__(self, seed: int, device: torch.device | str, nonce: int = 0) -> None:
+    def __init__(self, seed: int | None, device: torch.device | str, nonce: int | None = None) -> None:
         self.device = torch.device(device)
-        self.key = (ctypes.c_uint32 * 8)(*key_words_from_seed(seed))
+        words = entropy_key_words() if seed is None else key_words_from_seed(seed)
+        self.key = (ctypes.c_uint32 * 8)(*words)
+        if nonce is None:
+            nonce = 0 if seed is not None else secrets.randbits(64)
         self._nonce = [nonce & _MASK, (nonce >> 32) & _MASK, 0]
         self.counter = 0
 

@@ -33,9 +33,16 @@ from fhelium_amd.runtime import RnsRuntime
 class CkksEngine:
     def __init__(self, preset: Preset | str | CkksConfig = Preset.slots32768_scale40_levels34_int64,
                  *, device: str | torch.device = "cuda", ntt_mode: str = "fast",
-                 seed: int = 0x12345678, allow_secret_key_generation: bool = True,
-                 fused_keyswitch: bool = True) -> None:
+                 rng_seed: int | None = None, rng_nonce: int | None = None,
+                 allow_sk_gen: bool = True, fused_keyswitch: bool = True) -> None:
         self.config = preset if isinstance(preset, CkksConfig) else CkksConfig.from_preset(preset)
+        # rng_seed / rng_nonce fix the stream for tests and benchmarks only;
+        # left unset, the key and nonce come from the operating system's
+        # entropy (ckks_engine.py:72-74, 86-87).
+        if rng_seed is not None and type(rng_seed) is not int:
+            raise TypeError("rng_seed must be an integer or None")
+        if rng_nonce is not None and type(rng_nonce) is not int:
+            raise TypeError("rng_nonce must be an integer or None")
         self.runtime = RnsRuntime(self.config, device, ntt_mode)
         self.device = self.runtime.device
         self.layout = self.runtime.layout
@@ -43,10 +50,10 @@ class CkksEngine:
         # fused: eight-launch key switch (fh_keyswitch_fused); otherwise the
         # reference's per-digit op sequence.  Both are bit-identical.
         self.fused_keyswitch = fused_keyswitch
-        self.allow_secret_key_generation = allow_secret_key_generation
-        # Seeded ChaCha20 sampler on the device (fhelium_amd/csprng.py); key
-        # and ciphertext randomness never leaves HBM.
-        self.rng = Csprng(seed, self.device)
+        self.allow_sk_gen = allow_sk_gen
+        # ChaCha20 sampler on the device (fhelium_amd/csprng.py); key and
+        # ciphertext randomness never leaves HBM.
+        self.rng = Csprng(rng_seed, self.device, nonce=rng_nonce)
         self._moduli_cache: dict[tuple[int, ...], torch.Tensor] = {}
         self.secret_key: SecretKey | None = None
         self.relinearization_key: RelinearizationKey | None = None
@@ -123,7 +130,7 @@ class CkksEngine:
 
     def _require_secret(self) -> SecretKey:
         if self.secret_key is None:
-            if not self.allow_secret_key_generation:
+            if not self.allow_sk_gen:
                 raise RuntimeError("no secret key installed")
             self.generate_secret_key()
         return self.secret_key
@@ -174,7 +181,7 @@ class CkksEngine:
         step = RotationKey.canonical_step(rotation_step, self.config.N)
         key = self.rotation_keys.get(step)
         if key is None:
-            if not self.allow_secret_key_generation:
+            if not self.allow_sk_gen:
                 raise KeyError(f"no rotation key for step {step}")
             key = self.generate_rotation_key(step)
         return key
@@ -616,7 +623,7 @@ class CkksEngine:
         # d0, d1 stay in the NTT domain and seed the key switch's accumulators
         # (the result gains exactly d0, d1); relinearization and rescale run
         # in one key switch (ModDown writes the rescaled rows)
-        d = self.key_switcher.tensor_product_coefficient(a.data, b.data, a.level, ntt_d01=True)
-        out = self.key_switcher.key_switch_fused(d[2], key, a.level, rescale=True, seeds=d[:2])
+        seeds, d2 = self.key_switcher.tensor_product_seeded(a.data, b.data, a.level)
+        out = self.key_switcher.key_switch_fused(d2, key, a.level, rescale=True, seeds=seeds)
         dropped = self.config.moduli[self.runtime.row_start(a.level)]
         return Ciphertext(out, next_level, scale / float(dropped), "coefficient", "standard", "Q")

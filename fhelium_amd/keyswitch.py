@@ -27,6 +27,36 @@ SMALL_PRIME_LIMIT = 1 << 46
 INTEGER_PRIME_LIMIT = 1 << 60
 
 
+def _check_operand(name: str, t: torch.Tensor, batch: int, rows: int, n: int, device: torch.device) -> None:
+    """The kernels read exactly [batch, rows, n] int64 words on ``device``
+    through this operand's strides: anything else is rejected before a
+    launch."""
+
+    if t.dtype != torch.int64:
+        raise ValueError(f"{name} must be int64 residues, got {t.dtype}")
+    if t.device != device:
+        raise ValueError(f"{name} is on {t.device}, the component on {device}")
+    if t.dim() < 2 or tuple(t.shape[-2:]) != (rows, n) or t.numel() != batch * rows * n:
+        raise ValueError(f"{name} must hold [{batch}, {rows}, {n}] residues, got {tuple(t.shape)}")
+
+
+class AccumulatorSeeds:
+    """d0, d1 of an HMult front left in the NTT domain for the key switch.
+
+    ``data`` is [2, *batch, Q_level, N] int64 words in a private working
+    format, not residues: rows whose prime is below 2^46 hold the binary64
+    bit patterns of a b R^2 (signed, not reduced), the other rows a b R below
+    2q; within each 256-coefficient tile, coefficient 8t + k sits at position
+    t + 32k.  Only ``HybridKeySwitcher.key_switch_fused(seeds=...)`` reads
+    it."""
+
+    __slots__ = ("data", "level")
+
+    def __init__(self, data: torch.Tensor, level: int) -> None:
+        self.data = data
+        self.level = level
+
+
 class HybridKeySwitcher:
     def __init__(self, runtime: RnsRuntime) -> None:
         self.rt = runtime
@@ -136,12 +166,18 @@ class HybridKeySwitcher:
         n = cfg.N
         lq = cfg.num_q_primes - rt.row_start(level)
         lqp = lq + cfg.num_p_primes
+        if component.dim() < 2 or tuple(component.shape[-2:]) != (lq, n):
+            raise ValueError(f"key switch component must be [*batch, {lq}, {n}] at level {level}")
         batch_shape = component.shape[:-2]
         comp = component.reshape(-1, lq, n)
         b = comp.size(0)
+        dev = component.device
+        _check_operand("component", component, b, lq, n, dev)
+        for name, t in (("base0", base0), ("base1", base1)):
+            if t is not None:
+                _check_operand(name, t, b, lq, n, dev)
         table = self.digit_table(level)
         dnum = table.size(0)
-        dev = component.device
         ws_digits = torch.empty((b, lq, n), dtype=torch.int64, device=dev)
         ws_inter = torch.empty((b, dnum, lqp, n), dtype=torch.int64, device=dev)
         ws_acc = torch.empty((2, b, lqp, n), dtype=torch.int64, device=dev)
@@ -173,7 +209,12 @@ class HybridKeySwitcher:
             gal_sgn_ptr = self._address_table([t[1] for t in tables], dev).data_ptr()
         seed_ptrs = (None, None, None)
         if seeds is not None:
-            sv = seeds.reshape(2, -1, lq, n)
+            if not isinstance(seeds, AccumulatorSeeds):
+                raise TypeError("seeds must be the AccumulatorSeeds that tensor_product_seeded returns")
+            if seeds.level != level:
+                raise ValueError(f"seeds were made at level {seeds.level}, not {level}")
+            sv = seeds.data.reshape(2, -1, lq, n)
+            _check_operand("seeds", sv[0], b, lq, n, dev)
             if sv.size(1) != b or not sv.is_contiguous():
                 raise ValueError("seeds must be a contiguous [2, *batch, Q_level, N] tensor")
             seed_ptrs = (sv[0].data_ptr(), sv[1].data_ptr(), rt.seed_scale.data_ptr())
@@ -232,14 +273,24 @@ class HybridKeySwitcher:
             cache[level] = hit
         return hit[0].data_ptr(), hit[1].data_ptr(), hit[2]
 
-    def tensor_product_coefficient(self, a: torch.Tensor, b: torch.Tensor, level: int,
-                                   ntt_d01: bool = False) -> torch.Tensor:
+    def tensor_product_coefficient(self, a: torch.Tensor, b: torch.Tensor, level: int) -> torch.Tensor:
         """[3, *batch, Q_l, N] coefficient-domain d0, d1, d2 of two
         coefficient-domain ciphertexts a, b ([2, *batch, Q_l, N], canonical
         standard): NTT of both, tensor product and inverse NTT in one fused
-        pass (fh_ckks_hmult_tensor_coefficient, fast arithmetic).  With
-        ``ntt_d01`` only d2 returns to the coefficient domain; d0, d1 stay in
-        the NTT domain as ``key_switch_fused``'s ``seeds``."""
+        pass (fh_ckks_hmult_tensor_coefficient, fast arithmetic)."""
+
+        return self._tensor_product(a, b, level, False)
+
+    def tensor_product_seeded(self, a: torch.Tensor, b: torch.Tensor,
+                              level: int) -> tuple[AccumulatorSeeds, torch.Tensor]:
+        """The same product with only d2 ([*batch, Q_l, N], coefficient
+        domain) returned as residues; d0, d1 stay in the NTT domain as the
+        ``AccumulatorSeeds`` that ``key_switch_fused(seeds=...)`` consumes."""
+
+        out = self._tensor_product(a, b, level, True)
+        return AccumulatorSeeds(out[:2], level), out[2]
+
+    def _tensor_product(self, a: torch.Tensor, b: torch.Tensor, level: int, ntt_d01: bool) -> torch.Tensor:
 
         from fhelium_amd import _native as nat
 
@@ -250,6 +301,11 @@ class HybridKeySwitcher:
         lq = cfg.num_q_primes - start
         if a.shape != b.shape or a.dim() < 3 or a.size(0) != 2 or a.shape[-2:] != (lq, n):
             raise ValueError("tensor product operands must be [2, *batch, Q_level, N] of equal shape")
+        for name, t in (("a", a), ("b", b)):
+            if t.dtype != torch.int64:
+                raise ValueError(f"tensor product operand {name} must be int64 residues, got {t.dtype}")
+            if t.device != a.device:
+                raise ValueError("tensor product operands must share one device")
         batch_shape = a.shape[1:-2]
         av = a.reshape(2, -1, lq, n)
         bv = b.reshape(2, -1, lq, n)

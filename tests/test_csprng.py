@@ -51,3 +51,26 @@ def test_sampler_checkers_properties():
     u = C.uniform_mod(ks, q, 2, 100)
     assert u.shape == (2, 2, 100) and np.all(u[:, 0] < q[0]) and np.all(u >= 0)
     assert u[:, 1].max() > (1 << 59)
+
+
+def test_default_generators_draw_fresh_entropy():
+    """Without rng_seed the key and nonce come from the OS: two generators
+    differ, while an explicit seed reproduces the seeded key words."""
+
+    from fhelium_amd.csprng import Csprng, key_words_from_seed
+
+    a, b = Csprng(None, "cpu"), Csprng(None, "cpu")
+    assert list(a.key) != list(b.key)
+    assert a.nonce != b.nonce
+    s = Csprng(20260807, "cpu")
+    assert list(s.key) == key_words_from_seed(20260807)
+    assert s.nonce == (0, 0, 0)
+
+
+def test_engine_rejects_non_integer_seed():
+    import pytest
+
+    from fhelium_amd.engine import CkksEngine
+
+    with pytest.raises(TypeError):
+        CkksEngine(rng_seed="7", device="cpu")

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module", params=[12, 16])
 def eng(request):
-    e = CkksEngine(H.config(request.param), seed=71)
+    e = CkksEngine(H.config(request.param), rng_seed=71)
     e.generate_secret_key()
     return e
 

@@ -31,7 +31,7 @@ TOL_MULTIPLY = 3e-5
 @pytest.fixture(scope="module", params=["fast", "exact"])
 def small(request):
     cfg = H.config(12)
-    eng = CkksEngine(cfg, ntt_mode=request.param, seed=7)
+    eng = CkksEngine(cfg, ntt_mode=request.param, rng_seed=7)
     eng.generate_secret_key()
     eng.generate_relinearization_key()
     return eng, EngineRef(H.host_tables(12)), request.param
@@ -122,7 +122,7 @@ def test_state_errors(small):
 
 @pytest.fixture(scope="module")
 def full():
-    eng = CkksEngine(H.L34, ntt_mode="fast", seed=11)
+    eng = CkksEngine(H.L34, ntt_mode="fast", rng_seed=11)
     eng.generate_secret_key()
     eng.generate_relinearization_key()
     return eng
@@ -157,7 +157,7 @@ def test_basic_flow_l7():
     """README flow on preset slots8192-scale40-levels7 (logN=14): add, NTT,
     multiply, relinearize, rescale, rotate by 1, decrypt (atol 2e-5)."""
 
-    eng = CkksEngine(H.L7, seed=0x12345678)
+    eng = CkksEngine(H.L7, rng_seed=0x12345678)
     eng.generate_secret_key()
     x = np.linspace(-0.05, 0.05, 32)
     y = np.random.default_rng(20260807).uniform(-0.05, 0.05, 32)
@@ -182,7 +182,7 @@ def test_fused_keyswitch_matches_reference_sequence(mode, level):
     """fh_keyswitch_fused == the reference's per-digit op sequence, bit for bit,
     for a shared key and for per-entry keys, at logN=12 and logN=13."""
     for log_n in (12, 13):
-        eng = CkksEngine(H.config(log_n), ntt_mode=mode, seed=21)
+        eng = CkksEngine(H.config(log_n), ntt_mode=mode, rng_seed=21)
         eng.generate_secret_key()
         key = eng.generate_relinearization_key()
         rng = np.random.default_rng(level)
@@ -205,7 +205,7 @@ def test_fused_keyswitch_matches_reference_sequence(mode, level):
 
 @pytest.fixture(scope="module")
 def full_parity():
-    eng = CkksEngine(H.config(16), seed=17)
+    eng = CkksEngine(H.config(16), rng_seed=17)
     eng.generate_secret_key()
     eng.generate_relinearization_key()
     return eng, EngineRef(H.host_tables(16))

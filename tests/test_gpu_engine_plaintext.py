@@ -20,7 +20,7 @@ TOL = 2e-5
 
 @pytest.fixture(scope="module")
 def eng():
-    e = CkksEngine(H.config(12), seed=11)
+    e = CkksEngine(H.config(12), rng_seed=11)
     e.generate_secret_key()
     e.generate_relinearization_key()
     return e

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_captured_hmult_rotation_replays_bit_exact():
-    eng = CkksEngine(H.config(12), seed=31)
+    eng = CkksEngine(H.config(12), rng_seed=31)
     eng.generate_secret_key()
     eng.generate_relinearization_key()
     steps = [1, 5, 9]
@@ -48,7 +48,7 @@ def test_captured_hmult_rotation_replays_bit_exact():
 
 
 def test_replay_rejects_state_mismatch():
-    eng = CkksEngine(H.config(12), seed=32)
+    eng = CkksEngine(H.config(12), rng_seed=32)
     eng.generate_secret_key()
     x = np.zeros((2, eng.config.slots))
     a = eng.encrypt(x, level=2)

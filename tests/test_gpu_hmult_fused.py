@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module", params=[12, 16])
 def eng(request):
-    e = CkksEngine(H.config(request.param), seed=61)
+    e = CkksEngine(H.config(request.param), rng_seed=61)
     e.generate_secret_key()
     e.generate_relinearization_key()
     return e
@@ -90,8 +90,11 @@ def test_seeded_keyswitch_matches_bases(eng, level, rescale):
     ks = eng.key_switcher
     key = eng.relinearization_key
     d = ks.tensor_product_coefficient(a.data, b.data, level)
-    seeded = ks.tensor_product_coefficient(a.data, b.data, level, ntt_d01=True)
-    assert torch.equal(seeded[2], d[2])
+    seeds, d2 = ks.tensor_product_seeded(a.data, b.data, level)
+    assert torch.equal(d2, d[2])
     want = ks.key_switch_fused(d[2], key, level, base0=d[0], base1=d[1], rescale=rescale)
-    got = ks.key_switch_fused(seeded[2], key, level, rescale=rescale, seeds=seeded[:2])
+    got = ks.key_switch_fused(d2, key, level, rescale=rescale, seeds=seeds)
     assert torch.equal(got, want)
+    # plain residues are not seeds, and seeds carry their level
+    with pytest.raises(TypeError):
+        ks.key_switch_fused(d2, key, level, rescale=rescale, seeds=d[:2])

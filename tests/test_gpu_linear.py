@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("k,chunk,level", [(16, 32, 0), (21, 8, 5)])
 def test_diagonal_matvec(k, chunk, level):
-    eng = CkksEngine(H.config(12), seed=81)
+    eng = CkksEngine(H.config(12), rng_seed=81)
     eng.generate_secret_key()
     rng = np.random.default_rng(k)
     x = rng.uniform(-1, 1, eng.config.slots)

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_engine_state_round_trip(tmp_path):
-    a = CkksEngine(H.config(12), seed=51)
+    a = CkksEngine(H.config(12), rng_seed=51)
     a.generate_secret_key()
     a.generate_relinearization_key()
     a.rotation_key(7)
@@ -26,7 +26,7 @@ def test_engine_state_round_trip(tmp_path):
     for name, value, kw in (("sk", a.secret_key, {"allow_secret": True}), ("rlk", a.relinearization_key, {}),
                             ("rot", a.rotation_keys[7], {}), ("ct", ct, {})):
         save_value(value, tmp_path / f"{name}.safetensors", context=a, **kw)
-    b = CkksEngine(H.config(12), seed=99, allow_secret_key_generation=False)
+    b = CkksEngine(H.config(12), rng_seed=99, allow_sk_gen=False)
     b.set_secret_key(load_value(tmp_path / "sk.safetensors", context=b, device="cuda", expected_type=SecretKey))
     assert torch.equal(b.secret_key.coefficients.cpu(), a.secret_key.coefficients.cpu())
     b.set_relinearization_key(load_value(tmp_path / "rlk.safetensors", context=b, expected_type=RelinearizationKey))

@@ -23,7 +23,7 @@ TOL = WINDOW * 2e-5
 
 @pytest.fixture(scope="module")
 def setup():
-    eng = CkksEngine(H.config(12), seed=41)
+    eng = CkksEngine(H.config(12), rng_seed=41)
     eng.generate_secret_key()
     eng.generate_relinearization_key()
     prog = PolynomialWindowSum(eng, logistic_chebyshev(7), radius=4.0, window=WINDOW)
