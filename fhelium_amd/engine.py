@@ -27,6 +27,7 @@ from fhelium_amd.core import (Ciphertext, CkksStateError, InvalidScaleError, Key
                               MaximumLevelError, Plaintext, RelinearizationKey, RotationKey,
                               ScaleMismatchError, SecretKey)
 from fhelium_amd.keyswitch import HybridKeySwitcher
+from fhelium_amd.rotation import decompose_rotation_step
 from fhelium_amd.runtime import RnsRuntime
 
 
@@ -510,9 +511,25 @@ class CkksEngine:
             raise TypeError("rotation_step must be an integer")
         self._assert_rotation_ciphertext(ct)
         step = RotationKey.canonical_step(rotation_step, self.config.N)
+        result = ct.clone() if step == 0 else ct
+        for hop in self.rotation_path(step):
+            result = self.rotate_with_key(result, self.rotation_key(hop))
+        return result
+
+    def rotation_path(self, rotation_step: int) -> list[int]:
+        """Canonical steps whose rotations, applied in order, realise
+        ``rotation_step`` (ckks_engine.py:3726-3751): the step itself when
+        its key is installed or may be generated, otherwise the shortest
+        composition over the installed keys (rotation.decompose_rotation_step).
+        Raises KeyError when neither exists."""
+
+        step = RotationKey.canonical_step(rotation_step, self.config.N)
         if step == 0:
-            return ct.clone()
-        return self.rotate_with_key(ct, self.rotation_key(step))
+            return []
+        if step in self.rotation_keys or self.allow_sk_gen:
+            return [step]
+        return [RotationKey.canonical_step(s, self.config.N)
+                for s in decompose_rotation_step(step, self.config.slots, self.rotation_keys)]
 
     def rotate_with_key(self, ct: Ciphertext, key: RotationKey) -> Ciphertext:
         self._assert_rotation_ciphertext(ct)
@@ -554,8 +571,11 @@ class CkksEngine:
         self._assert_rotation_ciphertext(ct)
         if len(ct.batch_shape) != 1 or ct.batch_shape[0] != len(rotation_steps):
             raise ValueError("rotate_by_steps needs a [2, B, limb, N] ciphertext and B steps")
+        paths = [self.rotation_path(s) for s in rotation_steps]
+        if any(len(p) != 1 for p in paths):
+            return self._rotate_by_paths(ct, paths)
         rt = self.runtime
-        keys = [self.rotation_key(s) for s in rotation_steps]
+        keys = [self.rotation_key(p[0]) for p in paths]
         if self.fused_keyswitch:
             return self._switch_rotated_fused(ct, keys, [k.galois_element for k in keys])
         twice = rt.twice_modulus[rt.row_start(ct.level): self.config.num_q_primes]
@@ -564,6 +584,22 @@ class CkksEngine:
             idx, sgn = rt.coefficient_galois(key.galois_element)
             rotated[:, b] = K.apply_coefficient_galois_automorphism(ct.data[:, b], idx, sgn, twice)
         return self._switch_rotated(ct, rotated, keys)
+
+    def _rotate_by_paths(self, ct: Ciphertext, paths: list[list[int]]) -> Ciphertext:
+        """Per-entry composed rotations: hop k of every entry whose path has
+        one runs in one batched key switch; entry b's result equals its hops
+        applied one at a time."""
+
+        data = ct.data.clone()
+        for hop in range(max(len(p) for p in paths)):
+            active = [b for b, p in enumerate(paths) if len(p) > hop]
+            index = torch.tensor(active, dtype=torch.long, device=self.device)
+            sub = Ciphertext(data.index_select(1, index), ct.level, ct.scale, ct.polynomial_domain,
+                             ct.residue_representation, ct.modulus_basis, ct.context_id)
+            out = self.rotate_by_steps(sub, [paths[b][hop] for b in active])
+            data.index_copy_(1, index, out.data)
+        return Ciphertext(data, ct.level, ct.scale, ct.polynomial_domain, ct.residue_representation,
+                          ct.modulus_basis, ct.context_id)
 
     def rotate_many_with_keys(self, ct: Ciphertext, keys: list[RotationKey], *,
                               use_hoisting: bool = True) -> list[Ciphertext]:

@@ -228,3 +228,44 @@ def test_full_size_fused_hmult_and_rotation_bit_exact(full_parity, level):
     key = eng.rotation_key(5)
     rot = eng.rotate_by_step(got, 5)
     np.testing.assert_array_equal(_np(rot.data), ref.rotate(want, _np(key.data), key.galois_element, level + 1))
+
+
+def test_rotation_composed_from_installed_keys():
+    """Without a direct key and with key generation disallowed, rotate_by_step
+    composes the rotation from installed keys along the reference's shortest
+    walk (ckks_engine.py:3726-3751, core/rotation.py:57-110): residues equal
+    the oracle's rotations applied hop by hop, and each hop adds at most the
+    reference's key-switch tolerance."""
+    cfg = H.config(12)
+    eng = CkksEngine(cfg, rng_seed=11)
+    eng.generate_secret_key()
+    for s in (1, 4, -16, 64):
+        eng.rotation_key(s)
+    eng.allow_sk_gen = False
+    ref = EngineRef(H.host_tables(12))
+    level = 2
+    a, v = _fresh(eng, level, batch=(3,), seed=21)
+    assert eng.rotation_path(69) == [1, 4, 64]
+    assert eng.rotation_path(-11) == [-16, 1, 4]
+    assert eng.rotation_path(4) == [4]
+    with pytest.raises(KeyError):
+        eng.rotation_key(69)
+    for step in (69, -11):
+        out = eng.rotate_by_step(a, step)
+        want = _np(a.data)
+        for hop in eng.rotation_path(step):
+            key = eng.rotation_keys[hop]
+            want = ref.rotate(want, _np(key.data), key.galois_element, level)
+        np.testing.assert_array_equal(_np(out.data), want, err_msg=f"step {step}")
+        err = np.abs(eng.decrypt(out).real - np.roll(v, step, axis=-1)).max()
+        assert err < 3 * TOL_KEYSWITCH, (step, err)
+    # per-entry steps mixing direct and composed walks
+    steps = [69, 4, -11]
+    out = eng.rotate_by_steps(a, steps)
+    for b, s in enumerate(steps):
+        single = a.clone()
+        single.data = a.data[:, b: b + 1].contiguous()
+        np.testing.assert_array_equal(_np(out.data[:, b: b + 1]), _np(eng.rotate_by_step(single, s).data))
+    # a step 0 entry is left as it is
+    out0 = eng.rotate_by_steps(a, [0, 4, 69])
+    np.testing.assert_array_equal(_np(out0.data[:, 0]), _np(a.data[:, 0]))
