@@ -1332,29 +1332,56 @@ int side_stream(SideStream** out) {
   return kOk;
 }
 
-}  // namespace
-}  // namespace fh
+// Workspace of fh_keyswitch_fused_static: one array per device, 64 MiB.
+// Host code never uses this name as an address; it reads the device address
+// through hipGetSymbolAddress once per device (g_scratch_ptr).
+constexpr int64_t kScratchWords = int64_t{1} << 23;
+__device__ __align__(256) int64_t scratch[kScratchWords];
+int64_t* g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mutex;
 
-using namespace fh;
+// The calling thread's current device's copy of `scratch`.
+int scratch_address(int64_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("static key switch: no usable current device (%d)", dev);
+    return kLaunchFailed;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || ptr == nullptr) {
+      set_error("static key switch: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                hipGetErrorString(rc));
+      return kLaunchFailed;
+    }
+    g_scratch_ptr[dev] = static_cast<int64_t*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
 
-// One fused key switch of a batched component.  See include/fhelium_amd.h.
-FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp,
-                              int64_t level, int64_t log_n, int64_t small_rows,
-                              const int64_t* digit_table,
-                              int64_t digit_count,
-                              const int64_t* p_digit_table,
-                              const int64_t* p_inverse_q, fh_table2 rns_params, const void* ntt_rows,
-                              const int64_t* forward_twiddles, const int64_t* forward_twiddles_shoup,
-                              const int64_t* inverse_twiddles, const int64_t* inverse_twiddles_shoup,
-                              const int64_t* key, const int64_t* key_bases, int64_t key_stride_digit,
-                              int64_t key_stride_comp, int64_t key_stride_row,
-                              const int64_t* galois_index_tables, const int64_t* galois_sign_tables,
-                              int64_t* workspace_digits, int64_t* workspace_inter,
-                              int64_t* workspace_acc, fh_view3 base0, fh_view3 base1, fh_view3 out0,
-                              fh_view3 out1, int32_t mode, const int64_t* rescale_inverse,
-                              const int64_t* rescale_inverse_shoup, int64_t half_drop,
-                              const int64_t* seed0, const int64_t* seed1, const int64_t* seed_scale,
-                              void* stream) {
+// The eight-launch fused key switch.  With use_side_stream the integer rows'
+// ModUp and MAC run on the device's side stream concurrently with the
+// binary64 rows; without it every launch goes to the caller's stream.
+int keyswitch_fused_impl(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp,
+                         int64_t level, int64_t log_n, int64_t small_rows,
+                         const int64_t* digit_table,
+                         int64_t digit_count,
+                         const int64_t* p_digit_table,
+                         const int64_t* p_inverse_q, fh_table2 rns_params, const void* ntt_rows,
+                         const int64_t* forward_twiddles, const int64_t* forward_twiddles_shoup,
+                         const int64_t* inverse_twiddles, const int64_t* inverse_twiddles_shoup,
+                         const int64_t* key, const int64_t* key_bases, int64_t key_stride_digit,
+                         int64_t key_stride_comp, int64_t key_stride_row,
+                         const int64_t* galois_index_tables, const int64_t* galois_sign_tables,
+                         int64_t* workspace_digits, int64_t* workspace_inter,
+                         int64_t* workspace_acc, fh_view3 base0, fh_view3 base1, fh_view3 out0,
+                         fh_view3 out1, int32_t mode, const int64_t* rescale_inverse,
+                         const int64_t* rescale_inverse_shoup, int64_t half_drop,
+                         const int64_t* seed0, const int64_t* seed1, const int64_t* seed_scale,
+                         bool use_side_stream, void* stream) {
   FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "fused key switch supports logN 12..16");
   FH_CHECK_ARG(batch >= 1 && batch <= 32767, "fused key switch batch %lld out of range",
                (long long)batch);
@@ -1461,7 +1488,13 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
     FH_CHECK_LAUNCH("key switch tile pass + MAC");
     return kOk;
   };
-  {
+  if (!use_side_stream) {
+    for (int pass = 0; pass < 2; ++pass) {
+      int rc = launch_modup(pass, s);
+      if (rc == kOk) rc = launch_mac(pass, s);
+      if (rc != kOk) return rc;
+    }
+  } else {
     std::lock_guard<std::mutex> lock(g_side_mutex);
     SideStream* side = nullptr;
     int rc = side_stream(&side);
@@ -1550,6 +1583,82 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
   }
   FH_CHECK_LAUNCH("key switch moddown + add");
   return kOk;
+}
+
+}  // namespace
+}  // namespace fh
+
+using namespace fh;
+
+// One fused key switch of a batched component.  See include/fhelium_amd.h.
+FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp,
+                              int64_t level, int64_t log_n, int64_t small_rows,
+                              const int64_t* digit_table,
+                              int64_t digit_count,
+                              const int64_t* p_digit_table,
+                              const int64_t* p_inverse_q, fh_table2 rns_params, const void* ntt_rows,
+                              const int64_t* forward_twiddles, const int64_t* forward_twiddles_shoup,
+                              const int64_t* inverse_twiddles, const int64_t* inverse_twiddles_shoup,
+                              const int64_t* key, const int64_t* key_bases, int64_t key_stride_digit,
+                              int64_t key_stride_comp, int64_t key_stride_row,
+                              const int64_t* galois_index_tables, const int64_t* galois_sign_tables,
+                              int64_t* workspace_digits, int64_t* workspace_inter,
+                              int64_t* workspace_acc, fh_view3 base0, fh_view3 base1, fh_view3 out0,
+                              fh_view3 out1, int32_t mode, const int64_t* rescale_inverse,
+                              const int64_t* rescale_inverse_shoup, int64_t half_drop,
+                              const int64_t* seed0, const int64_t* seed1, const int64_t* seed_scale,
+                              void* stream) {
+  return keyswitch_fused_impl(component, batch, lq, lqp, level, log_n, small_rows, digit_table, digit_count,
+                              p_digit_table, p_inverse_q, rns_params, ntt_rows, forward_twiddles,
+                              forward_twiddles_shoup, inverse_twiddles, inverse_twiddles_shoup, key, key_bases,
+                              key_stride_digit, key_stride_comp, key_stride_row, galois_index_tables,
+                              galois_sign_tables, workspace_digits, workspace_inter, workspace_acc, base0, base1,
+                              out0, out1, mode, rescale_inverse, rescale_inverse_shoup, half_drop, seed0, seed1,
+                              seed_scale, /*use_side_stream=*/true, stream);
+}
+
+// fh_keyswitch_fused with its workspaces carved from the device's static
+// scratch array instead of caller memory.  See include/fhelium_amd.h.
+FH_API int fh_keyswitch_fused_static(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp,
+                                     int64_t level, int64_t log_n, int64_t small_rows,
+                                     const int64_t* digit_table, int64_t digit_count,
+                                     const int64_t* p_digit_table, const int64_t* p_inverse_q,
+                                     fh_table2 rns_params, const void* ntt_rows, const int64_t* forward_twiddles,
+                                     const int64_t* forward_twiddles_shoup, const int64_t* inverse_twiddles,
+                                     const int64_t* inverse_twiddles_shoup, const int64_t* key,
+                                     const int64_t* key_bases, int64_t key_stride_digit, int64_t key_stride_comp,
+                                     int64_t key_stride_row, const int64_t* galois_index_tables,
+                                     const int64_t* galois_sign_tables, fh_view3 base0, fh_view3 base1,
+                                     fh_view3 out0, fh_view3 out1, int32_t mode, const int64_t* rescale_inverse,
+                                     const int64_t* rescale_inverse_shoup, int64_t half_drop,
+                                     const int64_t* seed0, const int64_t* seed1, const int64_t* seed_scale,
+                                     void* stream) {
+  FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "fused key switch supports logN 12..16");
+  FH_CHECK_ARG(batch >= 1 && lq >= 1 && lqp > lq && digit_count >= 1 && digit_count <= 16,
+               "static key switch: bad batch, row or digit counts");
+  const int64_t n = int64_t{1} << log_n;
+  // digits [B, lq, N], ModUp intermediate [B, dnum, lqp, N], accumulators [2, B, lqp, N]
+  const int64_t digits_words = batch * lq * n;
+  const int64_t inter_words = batch * digit_count * lqp * n;
+  const int64_t acc_words = 2 * batch * lqp * n;
+  const int64_t need = digits_words + inter_words + acc_words;
+  FH_CHECK_ARG(need <= kScratchWords,
+               "static key switch needs %lld workspace words (batch %lld, %lld Q rows, %lld QP rows, %lld "
+               "digits, N %lld); the static scratch holds %lld (64 MiB)",
+               (long long)need, (long long)batch, (long long)lq, (long long)lqp, (long long)digit_count,
+               (long long)n, (long long)kScratchWords);
+  int64_t* base = nullptr;
+  const int rc = scratch_address(&base);
+  if (rc != kOk) return rc;
+  // Every workspace is fully written by the launch that produces it before
+  // any launch reads it, so the scratch needs no clearing between calls.
+  return keyswitch_fused_impl(component, batch, lq, lqp, level, log_n, small_rows, digit_table, digit_count,
+                              p_digit_table, p_inverse_q, rns_params, ntt_rows, forward_twiddles,
+                              forward_twiddles_shoup, inverse_twiddles, inverse_twiddles_shoup, key, key_bases,
+                              key_stride_digit, key_stride_comp, key_stride_row, galois_index_tables,
+                              galois_sign_tables, base, base + digits_words, base + digits_words + inter_words,
+                              base0, base1, out0, out1, mode, rescale_inverse, rescale_inverse_shoup, half_drop,
+                              seed0, seed1, seed_scale, /*use_side_stream=*/false, stream);
 }
 
 // Tensor product of two coefficient-domain ciphertexts, returned in the

@@ -143,7 +143,8 @@ class HybridKeySwitcher:
                          level: int, base0: torch.Tensor | None = None,
                          base1: torch.Tensor | None = None,
                          galois_elements: list[int] | None = None, rescale: bool = False,
-                         seeds: torch.Tensor | None = None) -> torch.Tensor:
+                         seeds: AccumulatorSeeds | None = None,
+                         static_workspace: bool = False) -> torch.Tensor:
         """Whole key switch in eight launches; returns [2, *batch, Q_level, N]
         holding (base0 + corr0, base1 + corr1) canonically added, or the bare
         corrections where a base is None.  With ``rescale`` the result is
@@ -157,7 +158,13 @@ class HybridKeySwitcher:
 
         With ``galois_elements`` (one per batch entry) the component and base0
         are read through entry b's coefficient automorphism, i.e. this is the
-        key switch of a rotation applied to the unrotated ciphertext."""
+        key switch of a rotation applied to the unrotated ciphertext.
+
+        With ``static_workspace`` the launches take their workspaces from the
+        library's per-device static scratch (fh_keyswitch_fused_static: 64 MiB,
+        one call at a time per device, every launch on the current stream)
+        instead of three PyTorch allocations; only the output is allocated
+        here."""
 
         from fhelium_amd import _native as nat
 
@@ -178,9 +185,13 @@ class HybridKeySwitcher:
                 _check_operand(name, t, b, lq, n, dev)
         table = self.digit_table(level)
         dnum = table.size(0)
-        ws_digits = torch.empty((b, lq, n), dtype=torch.int64, device=dev)
-        ws_inter = torch.empty((b, dnum, lqp, n), dtype=torch.int64, device=dev)
-        ws_acc = torch.empty((2, b, lqp, n), dtype=torch.int64, device=dev)
+        if static_workspace:
+            ws_args = ()
+        else:
+            ws_digits = torch.empty((b, lq, n), dtype=torch.int64, device=dev)
+            ws_inter = torch.empty((b, dnum, lqp, n), dtype=torch.int64, device=dev)
+            ws_acc = torch.empty((2, b, lqp, n), dtype=torch.int64, device=dev)
+            ws_args = (ws_digits.data_ptr(), ws_inter.data_ptr(), ws_acc.data_ptr())
         out_rows = lq - 1 if rescale else lq
         if rescale and lq < 2:
             raise ValueError("a fused rescale needs at least two Q rows")
@@ -231,12 +242,13 @@ class HybridKeySwitcher:
         small_rows = sum(small)
         if any(small[small_rows:]):
             raise ValueError("fused key switch needs the binary64 rows before the integer rows")
-        nat.call("fh_keyswitch_fused", nat.v3(comp), b, lq, lqp, start, cfg.logN, small_rows,
+        entry = "fh_keyswitch_fused_static" if static_workspace else "fh_keyswitch_fused"
+        nat.call(entry, nat.v3(comp), b, lq, lqp, start, cfg.logN, small_rows,
                  table.data_ptr(), dnum, self.p_digit_table().data_ptr(), rt.p_inverse_q.data_ptr(),
                  nat.t2(rt.rns_params), nt.rows.data_ptr(), nt.fwd.data_ptr(), nt.fwd_shoup.data_ptr(),
                  nt.inv.data_ptr(), nt.inv_shoup.data_ptr(), key_ptr, bases_ptr,
                  ref.stride(0), ref.stride(1), ref.stride(2), gal_idx_ptr, gal_sgn_ptr,
-                 ws_digits.data_ptr(), ws_inter.data_ptr(), ws_acc.data_ptr(), view(base0), view(base1),
+                 *ws_args, view(base0), view(base1),
                  nat.v3(out[0]), nat.v3(out[1]), rt.ntt._mode, *self._rescale_args(level, rescale),
                  *seed_ptrs, nat.stream_ptr(dev))
         if rec is not None:

@@ -229,6 +229,33 @@ int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lq
                        int64_t half_drop, const int64_t* seed0, const int64_t* seed1,
                        const int64_t* seed_scale, void* stream);
 
+/* fh_keyswitch_fused without caller workspaces: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The digits, ModUp and accumulator workspaces are carved from
+ * a static 64 MiB device array (one per GPU, held until the process ends),
+ * so batch * N * (lq + (digit_count + 2) * lqp) must not exceed 2^23 words
+ * (e.g. one ciphertext at every level for N <= 2^14); larger calls return
+ * an error before any launch.  Every launch runs on `stream` (no side
+ * stream).  All calls on one GPU share the array: calls that overlap in
+ * time on one GPU race, so serialize them (one stream, or events).  The
+ * workspaces need no clearing; each is fully written before it is read.
+ * Arguments are fh_keyswitch_fused's without the three workspaces, and the
+ * results are bit-identical to it.  The function itself allocates nothing. */
+int fh_keyswitch_fused_static(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp, int64_t level,
+                              int64_t log_n, int64_t small_rows, const int64_t* digit_table,
+                              int64_t digit_count, const int64_t* p_digit_table,
+                              const int64_t* p_inverse_q, fh_table2 rns_params, const void* ntt_rows,
+                              const int64_t* forward_twiddles, const int64_t* forward_twiddles_shoup,
+                              const int64_t* inverse_twiddles, const int64_t* inverse_twiddles_shoup,
+                              const int64_t* key, const int64_t* key_bases, int64_t key_stride_digit,
+                              int64_t key_stride_comp, int64_t key_stride_row,
+                              const int64_t* galois_index_tables, const int64_t* galois_sign_tables,
+                              fh_view3 base0, fh_view3 base1, fh_view3 out0, fh_view3 out1, int32_t mode,
+                              const int64_t* rescale_inverse, const int64_t* rescale_inverse_shoup,
+                              int64_t half_drop, const int64_t* seed0, const int64_t* seed1,
+                              const int64_t* seed_scale, void* stream);
+
 /* HMult front of the batching tier: d0 = a0 b0, d1 = a0 b1 + a1 b0,
  * d2 = a1 b1 of two coefficient-domain ciphertexts, returned in the
  * coefficient domain with canonical standard residues -- the composition
