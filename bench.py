@@ -174,38 +174,67 @@ def step_algorithmic_bytes(eng, groups) -> int:
     return total
 
 
-def cpu_baseline(eng, seconds_budget: float = 30.0):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(eng, seconds_budget: float = 20.0):
     """The oracle (CPU restatement of the reference kernels) on the host's
-    cores: one HMult + one rotation at level 16 of the same preset."""
+    cores: HMult (NTT conversion of both inputs, multiply, relinearize,
+    rescale) plus a rotation at levels 0, 8, 16, 24 and 32 of the same
+    preset, repeated while the budget lasts.  Threads: every core this
+    process may run on, capped by OMP_NUM_THREADS when the host sets it (the
+    share of a shared GPU host given to one GPU)."""
 
     from oracle.engine_ref import EngineRef
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = max(1, min(affinity, int(omp))) if omp and omp.isdigit() else max(1, affinity)
     os.environ["FH_ORACLE_THREADS"] = str(threads)
     ref = EngineRef(eng.runtime.host)
-    level = 16
+    levels = (0, 8, 16, 24, 32)
     rng = np.random.default_rng(1)
-    x = rng.uniform(-1, 1, size=(1, eng.config.slots))
-    a = eng.encrypt(x, level=level)
-    b = eng.encrypt(x, level=level)
     rlk = eng.relinearization_key.data.cpu().numpy()
     rot = eng.rotation_key(1)
     rotk = rot.data.cpu().numpy()
-    an, bn = a.data.cpu().numpy(), b.data.cpu().numpy()
+    inputs = {}
+    for level in levels:
+        x = rng.uniform(-1, 1, size=(1, eng.config.slots))
+        inputs[level] = (eng.encrypt(x, level=level).data.cpu().numpy(),
+                         eng.encrypt(x, level=level).data.cpu().numpy())
     ops = 0
+    per_level = {lvl: [] for lvl in levels}
     t0 = time.perf_counter()
+    rounds = 0
     while True:
-        na = ref.coefficient_domain_to_ntt_domain(an, level)
-        nb = ref.coefficient_domain_to_ntt_domain(bn, level)
-        r = ref.rescale(ref.relinearize(ref.multiply(na, nb, level), rlk, level), level)
-        ref.rotate(r, rotk, rot.galois_element, level + 1)
-        ops += 2
-        if time.perf_counter() - t0 > seconds_budget / 3 or ops >= 6:
+        for level in levels:
+            an, bn = inputs[level]
+            t1 = time.perf_counter()
+            na = ref.coefficient_domain_to_ntt_domain(an, level)
+            nb = ref.coefficient_domain_to_ntt_domain(bn, level)
+            r = ref.rescale(ref.relinearize(ref.multiply(na, nb, level), rlk, level), level)
+            ref.rotate(r, rotk, rot.galois_element, level + 1)
+            per_level[level].append(time.perf_counter() - t1)
+            ops += 2
+        rounds += 1
+        if time.perf_counter() - t0 > seconds_budget:
             break
     dt = time.perf_counter() - t0
     return {"value": ops / dt, "unit": "ops/s", "cores": threads, "kind": "port",
-            "sample": f"{ops // 2} x (HMult at level {level} + rotation at level {level + 1}), "
-                      "logN=16 L34, oracle C restatement with OpenMP over RNS rows"}
+            "sample": f"{rounds} round(s) of HMult + rotation at each of levels {list(levels)} "
+                      "(logN=16, L34): the oracle's C restatement of the reference CPU kernels, "
+                      "OpenMP over RNS rows",
+            "ops_per_s_by_level": {str(k): 2 / float(np.median(v)) for k, v in per_level.items()},
+            "os_cpu_count": os.cpu_count(), "affinity_cpus": affinity, "omp_num_threads": omp,
+            "torch_num_threads": torch.get_num_threads(), "cpu_model": _cpu_model()}
 
 
 def ntt_measure(eng, reps: int = 5) -> dict:
@@ -461,6 +490,10 @@ def main() -> int:
             "frac": (ks_achieved / HBM_PEAK_GBS) if ks_achieved else None,
             "traffic": (KS_TRAFFIC_BYTES_PER_ALG_BYTE * ks_bytes / ks_calls
                         if ks_calls and KS_TRAFFIC_BYTES_PER_ALG_BYTE else None),
+            "traffic_source": "estimate: measured HBM bytes per algorithmic byte "
+                              f"({KS_TRAFFIC_BYTES_PER_ALG_BYTE}) from the rocprofv3 --pmc FETCH_SIZE and "
+                              "WRITE_SIZE passes in profiles/keyswitch_fused_traffic.md, times this "
+                              "run's algorithmic bytes",
             "calls": ks_calls,
             "avg_call_ms": ks_ms / ks_calls if ks_calls else None,
             "algorithmic_bytes_per_call": (ks_bytes / ks_calls) if ks_calls else None,

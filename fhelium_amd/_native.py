@@ -86,6 +86,9 @@ SIGNATURES: dict[str, list] = {
                                   _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr,
                                   fh_view3, fh_view3, fh_view3, fh_view3, _i32, _ptr, _ptr, _i64, _ptr, _ptr,
                                   _ptr, _ptr],
+    "fh_keyswitch_hoisted": [fh_view3, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, _i64, _ptr, _ptr, fh_table2,
+                             _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr,
+                             _ptr, fh_view3, fh_view3, fh_view3, _i32, _ptr],
     "fh_chacha20_keystream": [_ptr, _ptr, ctypes.c_uint32, _i64, _ptr, _ptr],
     "fh_sample_uniform_mod": [_ptr, _ptr, ctypes.c_uint32, _ptr, _i64, _i64, _i64, _ptr, _ptr],
     "fh_sample_ternary": [_ptr, _ptr, ctypes.c_uint32, _i64, _ptr, _ptr],

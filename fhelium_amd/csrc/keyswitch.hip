@@ -110,6 +110,11 @@ struct KsArgs {
   const int64_t* seed0;
   const int64_t* seed1;
   const int64_t* seed_scale;  // [Q0] P mod q (binary64 rows), P R mod q (integer rows)
+  // Hoisted rotations (null otherwise): [B, 2] (g, g^{-1} mod 2N) per entry.
+  // The decompose and ModUp launches then see one input entry, and every MAC
+  // entry reads that entry's intermediate through the NTT-domain Galois map
+  // of its own g.
+  const int64_t* hoist_galois;
 };
 
 // Block i runs on XCD i % 8; give each XCD one contiguous range of work
@@ -565,7 +570,47 @@ __device__ __forceinline__ int mac_e3(int t, int k) { return 8 * t + k; }
 // stage st uses 2^st twiddles, stored after those of the earlier stages.
 __device__ __forceinline__ int mac_tw_slot(int st, int e) { return (1 << st) - 1 + (e >> (8 - st)); }
 
-template <class P>
+// NTT-domain Galois automorphism on tiles (hybrid_keyswitch.py:506-528):
+// out[k] = in[br(((2 br(k) + 1) g mod 2N - 1) / 2)].  With k = 256 r + i the
+// source of every element of destination tile r lies in one source tile r',
+// and inside the tile the map is affine in bit-reversed coordinates:
+// br8(src_i) = (g br8(i) + c_r) mod 256, c_r = ((2 br(r) + 1) g mod 2N) >> (L-7).
+// In the MAC's slot layout (thread t holds elements 8t..8t+7) the elements
+// of destination thread t all come from one source thread.  A hoisted MAC
+// block is source-stationary: it reads source tile r' and finds the
+// destination tile, and for its thread t the destination element of each
+// slot k.
+__device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) { return __brev(x) >> (32 - bits); }
+
+struct HoistSlot {
+  int dst_tile;   // destination tile of source tile r'
+  int dst_base;   // 8 * destination thread of source thread t
+  int h;          // carry of the destination thread's slot map
+};
+
+__device__ __forceinline__ HoistSlot hoist_slot(int r_src, int t, uint32_t g, uint32_t ginv, int log_n) {
+  const int l8 = log_n - kTileLog;
+  const uint32_t low_mask = (1u << (log_n - 7)) - 1;  // mod 2^(L-7)
+  const uint32_t e_src = 2 * brev_bits(static_cast<uint32_t>(r_src), l8) + 1;
+  const uint32_t e_dst = (e_src * ginv) & low_mask;
+  HoistSlot hs;
+  hs.dst_tile = static_cast<int>(brev_bits((e_dst - 1) >> 1, l8));
+  const uint32_t u = ((2 * brev_bits(static_cast<uint32_t>(hs.dst_tile), l8) + 1) * g) & ((2u << log_n) - 1);
+  const uint32_t c = u >> (log_n - 7);
+  const uint32_t td = brev_bits(((brev_bits(static_cast<uint32_t>(t), 5) - c) * ginv) & 31u, 5);
+  hs.dst_base = static_cast<int>(8 * td);
+  hs.h = static_cast<int>((((g * brev_bits(td, 5) + c) & 255u) >> 5));
+  return hs;
+}
+// Destination slot (0..7) of source slot k.
+__device__ __forceinline__ int hoist_dst_slot(int k, int h, uint32_t ginv) {
+  return static_cast<int>(brev_bits(((brev_bits(static_cast<uint32_t>(k), 3) - static_cast<uint32_t>(h)) * ginv) & 7u, 3));
+}
+
+// kHoist: the block's tiles are source tiles of a hoisted rotation's
+// NTT-domain Galois map (see hoist_slot); otherwise source and destination
+// tiles coincide.
+template <bool kHoist, class P>
 __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g, int64_t j,
                                           int64_t b, int64_t* lds) {
   using V = typename P::V;
@@ -581,6 +626,26 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
   int64_t* tww = lds + kMacExchange + m * kTile;
   int64_t* twp = tww + kMacTiles * kTile;  // integer rows only
   int64_t* key_digit = lds + kMacExchange + kMacTiles * kTile * (kF64 ? 1 : 2);
+  // Destination tile of the accumulators and, packed 3 bits per slot, the
+  // destination slot of each of this thread's 8 source slots.
+  int r_dst = r;
+  int dst_base = 8 * t;
+  uint32_t slot_map = 0xFAC688u;  // slots 0..7 in order
+  const int64_t b_in = kHoist ? 0 : b;  // hoisted entries share one intermediate
+  if constexpr (kHoist) {
+    const uint32_t hg = static_cast<uint32_t>(a.hoist_galois[2 * b]);
+    const uint32_t hginv = static_cast<uint32_t>(a.hoist_galois[2 * b + 1]);
+    const HoistSlot hs = hoist_slot(r, t, hg, hginv, a.log_n);
+    r_dst = hs.dst_tile;
+    dst_base = hs.dst_base;
+    slot_map = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) slot_map |= static_cast<uint32_t>(hoist_dst_slot(k, hs.h, hginv)) << (3 * k);
+  }
+  auto key_slot = [&](int k) {
+    if constexpr (kHoist) return dst_base + static_cast<int>((slot_map >> (3 * k)) & 7u);
+    return mac_e3(t, k);
+  };
   {
     const int64_t* w_row = a.fwd_w + grow * n;
     const int64_t* wp_row = a.fwd_wp + grow * n;
@@ -598,7 +663,8 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     return p.tw_make(tww[slot], kF64 ? 0 : twp[slot]);
   };
   Global<int64_t>* key = global(a.key_bases ? reinterpret_cast<const int64_t*>(a.key_bases[b]) : a.key);
-  const int64_t tile_off = static_cast<int64_t>(r) * kTile;
+  const int64_t tile_off = static_cast<int64_t>(r) * kTile;       // source tile
+  const int64_t dst_off = static_cast<int64_t>(r_dst) * kTile;    // key and accumulators
   double facc0[8], facc1[8];
   uint64_t uacc0[8], uacc1[8];
   WideAcc wacc0[8], wacc1[8];  // integer rows: exact sums, one REDC after the digits
@@ -607,11 +673,11 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     facc0[k] = facc1[k] = 0.0;
     wacc0[k] = wacc1[k] = WideAcc{0, 0, 0};
   }
-  const int64_t* tile = a.inter + ((b * a.dnum) * a.lqp + j) * n + tile_off;
+  const int64_t* tile = a.inter + ((b_in * a.dnum) * a.lqp + j) * n + tile_off;
   const int64_t digit_stride = a.lqp * n;
 #pragma nounroll
   for (int64_t di = 0; di < a.dnum; ++di, tile += digit_stride) {
-    Global<int64_t>* k0 = key + key_digit[di] * a.key_sd + grow * a.key_sr + tile_off;
+    Global<int64_t>* k0 = key + key_digit[di] * a.key_sd + grow * a.key_sr + dst_off;
     Global<int64_t>* k1 = k0 + a.key_sc;
     V v[8];
     int64_t w0[8], w1[8];
@@ -619,7 +685,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       // Eight consecutive coefficients 8t..8t+7 of the tile are contiguous
       // in the packed block layout (one 32-byte and one 16-byte load); an
       // LDS pass moves them to the round-1 ownership.
-      const PackedInter pk = packed_slab(a.inter + (b * a.dnum + di) * a.lqp * n, a.small_rows, n);
+      const PackedInter pk = packed_slab(a.inter + (b_in * a.dnum + di) * a.lqp * n, a.small_rows, n);
       const uint32_t idx = static_cast<uint32_t>(j * n + packed_index(r, 8 * t));
       const uint4 l0 = *reinterpret_cast<const uint4*>(pk.lo + idx);
       const uint4 l1 = *reinterpret_cast<const uint4*>(pk.lo + idx + 4);
@@ -644,8 +710,8 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     if constexpr (kF64) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        w0[k] = k0[mac_e3(t, k)];
-        w1[k] = k1[mac_e3(t, k)];
+        w0[k] = k0[key_slot(k)];
+        w1[k] = k1[key_slot(k)];
       }
     }
     // Round 1: tile stages 0..2 pair element bits 7..5 = bits 2..0 of k.
@@ -697,8 +763,8 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
         mac_f64(facc0[k], v[k], w0[k], row.qd, row.qinv);
         mac_f64(facc1[k], v[k], w1[k], row.qd, row.qinv);
       } else {
-        w0[k] = k0[mac_e3(t, k)];
-        w1[k] = k1[mac_e3(t, k)];
+        w0[k] = k0[key_slot(k)];
+        w1[k] = k1[key_slot(k)];
         wide_mac(wacc0[k], v[k], static_cast<uint64_t>(w0[k]));
         wide_mac(wacc1[k], v[k], static_cast<uint64_t>(w1[k]));
       }
@@ -708,7 +774,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
   // scaling) join the Q rows' sums, so that ModDown's exact floor division
   // returns floor(X / P) + d directly.  Thread t holds coefficients 8t + k,
   // stored at tile positions t + 32k (the HMult front's seed layout).
-  if (a.seed0 != nullptr && j < a.lq) {
+  if (!kHoist && a.seed0 != nullptr && j < a.lq) {
     const int64_t off = (b * a.lq + j) * n + tile_off + t;
     if constexpr (kF64) {
       const double sc = static_cast<double>(a.seed_scale[grow]);
@@ -738,6 +804,33 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       uacc1[k] = wide_redc62(wacc1[k], row.q, row.neg_qinv64);
     }
   }
+  // Hoisted: source slot k of this thread accumulated destination element
+  // dst_base + slot k; move the sums to the destination layout (thread t
+  // holding elements 8t..8t+7 of tile r_dst) through the exchange buffer.
+  if constexpr (kHoist) {
+    auto to_dst = [&](auto (&acc)[8]) {
+      tile_sync();  // earlier readers of the exchange buffer are done
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        int64_t bits;
+        __builtin_memcpy(&bits, &acc[k], sizeof(bits));
+        sh[tile_pad(key_slot(k))] = bits;
+      }
+      tile_sync();
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int64_t bits = sh[tile_pad(mac_e3(t, k))];
+        __builtin_memcpy(&acc[k], &bits, sizeof(bits));
+      }
+    };
+    if constexpr (kF64) {
+      to_dst(facc0);
+      to_dst(facc1);
+    } else {
+      to_dst(uacc0);
+      to_dst(uacc1);
+    }
+  }
   // The first 8 stages of the accumulators' inverse NTT act inside this
   // tile: run them here (fast arithmetic, as the rest of the inverse) and
   // write the working-format result; only the column pass remains.
@@ -749,7 +842,7 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       // stage st owns slots [256 - 2^(8-st), 256 - 2^(7-st)); group g = e >> (st+1)
       const int st = 7 - (31 - __builtin_clz(kTile - 1 - idx));
       const int g = idx - (kTile - (kTile >> st));
-      const int64_t gi = (n >> (st + 1)) + (static_cast<int64_t>(r) << (7 - st)) + g;
+      const int64_t gi = (n >> (st + 1)) + (static_cast<int64_t>(r_dst) << (7 - st)) + g;
       tww[idx] = w_row[gi];
       if constexpr (!kF64) twp[idx] = wp_row[gi];
     }
@@ -812,15 +905,15 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     const PackedInter pk1 = packed_slab(a.acc + (1 * a.batch + b) * a.lqp * n, a.small_rows, n);
     const int64_t row_off = j * n;
     inverse_tile(pi, facc0, [&](int e, double x) {
-      packed_store(pk0, row_off + packed_index(r, e), f64_to_u64(pi.canon(x)));
+      packed_store(pk0, row_off + packed_index(r_dst, e), f64_to_u64(pi.canon(x)));
     });
     inverse_tile(pi, facc1, [&](int e, double x) {
-      packed_store(pk1, row_off + packed_index(r, e), f64_to_u64(pi.canon(x)));
+      packed_store(pk1, row_off + packed_index(r_dst, e), f64_to_u64(pi.canon(x)));
     });
   } else {
     const U64Fast pi(row);
-    int64_t* out0 = a.acc + ((0 * a.batch + b) * a.lqp + j) * n + tile_off;
-    int64_t* out1 = a.acc + ((1 * a.batch + b) * a.lqp + j) * n + tile_off;
+    int64_t* out0 = a.acc + ((0 * a.batch + b) * a.lqp + j) * n + dst_off;
+    int64_t* out1 = a.acc + ((1 * a.batch + b) * a.lqp + j) * n + dst_off;
     inverse_tile(pi, uacc0, [&](int e, uint64_t x) { out0[e] = static_cast<int64_t>(x); });
     inverse_tile(pi, uacc1, [&](int e, uint64_t x) { out1[e] = static_cast<int64_t>(x); });
   }
@@ -834,9 +927,9 @@ constexpr int kMacTilesPerBlock = kMacTiles;
 
 // One launch per arithmetic path (binary64 rows, then the integer rows) so
 // each kernel is register-allocated for its own path only.
-template <int MODE, bool kSmall>
-// Register budgets: 4 waves per SIMD for the binary64 rows (a few spilled
-// dwords cost less than the lost occupancy), 2+ for the integer rows.
+template <int MODE, bool kSmall, bool kHoist>
+// Register budgets: 3 waves per SIMD for the binary64 rows (no spills at 156
+// VGPRs), 2+ for the integer rows.
 __global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 3 : 2))) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
                                                          int64_t j_count) {
   __shared__ int64_t lds[mac_lds_words<kSmall>()];
@@ -849,12 +942,12 @@ __global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmal
   const NttRow& row = a.rows[a.level + j];
   if constexpr (kSmall) {
     if constexpr (MODE == 1) {
-      tiles_mac(F64Fast(row), a, g, j, b, lds);
+      tiles_mac<kHoist>(F64Fast(row), a, g, j, b, lds);
     } else {
-      tiles_mac(F64Exact(row), a, g, j, b, lds);
+      tiles_mac<kHoist>(F64Exact(row), a, g, j, b, lds);
     }
   } else {
-    tiles_mac(U64Fast(row), a, g, j, b, lds);
+    tiles_mac<kHoist>(U64Fast(row), a, g, j, b, lds);
   }
 }
 
@@ -1381,7 +1474,7 @@ int keyswitch_fused_impl(fh_view3 component, int64_t batch, int64_t lq, int64_t 
                          fh_view3 out1, int32_t mode, const int64_t* rescale_inverse,
                          const int64_t* rescale_inverse_shoup, int64_t half_drop,
                          const int64_t* seed0, const int64_t* seed1, const int64_t* seed_scale,
-                         bool use_side_stream, void* stream) {
+                         const int64_t* hoist_galois, bool use_side_stream, void* stream) {
   FH_CHECK_ARG(log_n >= 12 && log_n <= 16, "fused key switch supports logN 12..16");
   FH_CHECK_ARG(batch >= 1 && batch <= 32767, "fused key switch batch %lld out of range",
                (long long)batch);
@@ -1430,13 +1523,26 @@ int keyswitch_fused_impl(fh_view3 component, int64_t batch, int64_t lq, int64_t 
   a.seed0 = seed0;
   a.seed1 = seed1;
   a.seed_scale = seed_scale;
+  a.hoist_galois = hoist_galois;
+  // Hoisted rotations decompose and extend the one input entry once; the
+  // MAC and everything after it run per entry.
+  const bool hoist = hoist_galois != nullptr;
+  FH_CHECK_ARG(!hoist || (seed0 == nullptr && rescale_inverse == nullptr),
+               "hoisted rotations take no accumulator seeds and no fused rescale");
+  KsArgs a_in = a;
+  if (hoist) {
+    a_in.batch = 1;
+    a_in.gal_idx = nullptr;
+    a_in.gal_sign = nullptr;
+  }
+  const int64_t batch_in = hoist ? 1 : batch;
 
   FH_CHECK_ARG(small_rows >= 0 && small_rows <= lqp, "bad binary64 row count");
   const int s1 = static_cast<int>(log_n) - kTileLog;
   // n >= 4096 is a multiple of the block size; grids are flat and remapped.
-  const int64_t dec_blocks = (n / kBlock) * digit_count * batch;
+  const int64_t dec_blocks = (n / kBlock) * digit_count * batch_in;
   FH_CHECK_ARG(dec_blocks <= 0x7fffffff, "key switch grid too large");
-  hipLaunchKernelGGL(k_ks_decompose, dim3(static_cast<uint32_t>(dec_blocks)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_ks_decompose, dim3(static_cast<uint32_t>(dec_blocks)), dim3(kBlock), 0, s, a_in);
   FH_CHECK_LAUNCH("key switch decompose");
 
   const int T = (1 << s1) / 16;
@@ -1448,17 +1554,17 @@ int keyswitch_fused_impl(fh_view3 component, int64_t batch, int64_t lq, int64_t 
     const int64_t j_begin = pass == 0 ? 0 : small_rows;
     const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
     if (j_count == 0) return kOk;
-    const int64_t modup_blocks = (kTile / C) * j_count * digit_count * batch;
+    const int64_t modup_blocks = (kTile / C) * j_count * digit_count * batch_in;
     FH_CHECK_ARG(modup_blocks <= 0x7fffffff, "key switch grid too large");
     const dim3 gcol(static_cast<uint32_t>(modup_blocks));
 #define FH_MODUP_CASE(S)                                                                           \
   case S:                                                                                          \
     if (pass == 1) {                                                                               \
-      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, false>), gcol, dim3(kBlock), 0, st, a, j_begin, j_count); \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, false>), gcol, dim3(kBlock), 0, st, a_in, j_begin, j_count); \
     } else if (mode == 1) {                                                                        \
-      hipLaunchKernelGGL((k_ks_modup_columns<S, 1, true>), gcol, dim3(kBlock), 0, st, a, j_begin, j_count);  \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 1, true>), gcol, dim3(kBlock), 0, st, a_in, j_begin, j_count);  \
     } else {                                                                                       \
-      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, true>), gcol, dim3(kBlock), 0, st, a, j_begin, j_count);  \
+      hipLaunchKernelGGL((k_ks_modup_columns<S, 0, true>), gcol, dim3(kBlock), 0, st, a_in, j_begin, j_count);  \
     }                                                                                              \
     break;
     switch (s1) {
@@ -1478,13 +1584,20 @@ int keyswitch_fused_impl(fh_view3 component, int64_t batch, int64_t lq, int64_t 
     const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
     if (j_count == 0) return kOk;
     const dim3 grid(static_cast<uint32_t>(groups * j_count * batch));
-    if (pass == 1) {
-      hipLaunchKernelGGL((k_ks_tiles_mac<0, false>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);
-    } else if (mode == 1) {
-      hipLaunchKernelGGL((k_ks_tiles_mac<1, true>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);
-    } else {
-      hipLaunchKernelGGL((k_ks_tiles_mac<0, true>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);
+#define FH_MAC_LAUNCH(H)                                                                                       \
+    if (pass == 1) {                                                                                          \
+      hipLaunchKernelGGL((k_ks_tiles_mac<0, false, H>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count); \
+    } else if (mode == 1) {                                                                                   \
+      hipLaunchKernelGGL((k_ks_tiles_mac<1, true, H>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);  \
+    } else {                                                                                                  \
+      hipLaunchKernelGGL((k_ks_tiles_mac<0, true, H>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);  \
     }
+    if (hoist) {
+      FH_MAC_LAUNCH(true)
+    } else {
+      FH_MAC_LAUNCH(false)
+    }
+#undef FH_MAC_LAUNCH
     FH_CHECK_LAUNCH("key switch tile pass + MAC");
     return kOk;
   };
@@ -1614,7 +1727,34 @@ FH_API int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int
                               key_stride_digit, key_stride_comp, key_stride_row, galois_index_tables,
                               galois_sign_tables, workspace_digits, workspace_inter, workspace_acc, base0, base1,
                               out0, out1, mode, rescale_inverse, rescale_inverse_shoup, half_drop, seed0, seed1,
-                              seed_scale, /*use_side_stream=*/true, stream);
+                              seed_scale, /*hoist_galois=*/nullptr, /*use_side_stream=*/true, stream);
+}
+
+// Hoisted rotations of one ciphertext by several keys.  See include/fhelium_amd.h.
+FH_API int fh_keyswitch_hoisted(fh_view3 component, int64_t entries, int64_t lq, int64_t lqp, int64_t level,
+                                int64_t log_n, int64_t small_rows, const int64_t* digit_table,
+                                int64_t digit_count, const int64_t* p_digit_table, const int64_t* p_inverse_q,
+                                fh_table2 rns_params, const void* ntt_rows, const int64_t* forward_twiddles,
+                                const int64_t* forward_twiddles_shoup, const int64_t* inverse_twiddles,
+                                const int64_t* inverse_twiddles_shoup, const int64_t* key_bases,
+                                int64_t key_stride_digit, int64_t key_stride_comp, int64_t key_stride_row,
+                                const int64_t* galois_elements, const int64_t* galois_index_tables,
+                                const int64_t* galois_sign_tables, int64_t* workspace_digits,
+                                int64_t* workspace_inter, int64_t* workspace_acc, fh_view3 base0, fh_view3 out0,
+                                fh_view3 out1, int32_t mode, void* stream) {
+  FH_CHECK_ARG(galois_elements != nullptr && key_bases != nullptr && galois_index_tables != nullptr &&
+                   galois_sign_tables != nullptr && base0.data != nullptr,
+               "hoisted rotations need per-entry keys, Galois elements, Galois tables and the base c0");
+  fh_view3 c0 = base0;
+  c0.stride_batch = 0;  // every entry rotates the same c0
+  const fh_view3 none{nullptr, 0, 0, 0};
+  return keyswitch_fused_impl(component, entries, lq, lqp, level, log_n, small_rows, digit_table, digit_count,
+                              p_digit_table, p_inverse_q, rns_params, ntt_rows, forward_twiddles,
+                              forward_twiddles_shoup, inverse_twiddles, inverse_twiddles_shoup, nullptr, key_bases,
+                              key_stride_digit, key_stride_comp, key_stride_row, galois_index_tables,
+                              galois_sign_tables, workspace_digits, workspace_inter, workspace_acc, c0, none,
+                              out0, out1, mode, nullptr, nullptr, 0, nullptr, nullptr, nullptr, galois_elements,
+                              /*use_side_stream=*/true, stream);
 }
 
 // fh_keyswitch_fused with its workspaces carved from the device's static
@@ -1658,7 +1798,7 @@ FH_API int fh_keyswitch_fused_static(fh_view3 component, int64_t batch, int64_t 
                               key_stride_digit, key_stride_comp, key_stride_row, galois_index_tables,
                               galois_sign_tables, base, base + digits_words, base + digits_words + inter_words,
                               base0, base1, out0, out1, mode, rescale_inverse, rescale_inverse_shoup, half_drop,
-                              seed0, seed1, seed_scale, /*use_side_stream=*/false, stream);
+                              seed0, seed1, seed_scale, /*hoist_galois=*/nullptr, /*use_side_stream=*/false, stream);
 }
 
 // Tensor product of two coefficient-domain ciphertexts, returned in the

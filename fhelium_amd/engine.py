@@ -605,30 +605,118 @@ class CkksEngine:
                               use_hoisting: bool = True) -> list[Ciphertext]:
         """Hoisted rotations: one ModUp + NTT of c1 shared by every key
         (ckks_engine.py:2298-2331, 3767-3820).  As in the reference, fewer
-        than two non-zero steps fall back to direct rotations."""
+        than two non-zero steps fall back to direct rotations.  Every result
+        owns its storage."""
 
         self._assert_rotation_ciphertext(ct)
         for key in keys:
             if RotationKey.canonical_step(key.rotation_step, self.config.N) != key.rotation_step:
                 raise ValueError("rotation key must carry a canonical signed step")
-        if not use_hoisting or sum(k.rotation_step != 0 for k in keys) < 2:
-            return [ct.clone() if k.rotation_step == 0 else self.rotate_with_key(ct, k) for k in keys]
+        return self._apply_rotation_key_sequence(ct, [(k.rotation_step, k) for k in keys],
+                                                 use_hoisting=use_hoisting)
+
+    def rotate_many_by_steps(self, ct: Ciphertext, rotation_steps: list[int], *,
+                             use_hoisting: bool = True) -> list[Ciphertext]:
+        """Rotations by ordered signed steps with engine-owned keys
+        (ckks_engine.py:2247-2297): steps with a direct key (installed or
+        generated) share one hoisted decomposition; a step that needs a
+        composed key path is evaluated on its own."""
+
+        self._assert_rotation_ciphertext(ct)
+        for s in rotation_steps:
+            if not isinstance(s, int):
+                raise TypeError("All rotation steps must be integers")
+        steps = [RotationKey.canonical_step(s, self.config.N) for s in rotation_steps]
+        paths = [self.rotation_path(s) for s in steps]
+        if not use_hoisting:
+            return [self._apply_path(ct, p) for p in paths]
+        direct = [(i, s, self.rotation_key(p[0]) if p else None)
+                  for i, (s, p) in enumerate(zip(steps, paths)) if len(p) <= 1]
+        outputs = {i: self._apply_path(ct, p) for i, p in enumerate(paths) if len(p) > 1}
+        results = self._apply_rotation_key_sequence(ct, [(s, k) for _, s, k in direct], use_hoisting=True)
+        outputs.update(zip([i for i, _, _ in direct], results))
+        return [outputs[i] for i in range(len(paths))]
+
+    def _apply_path(self, ct: Ciphertext, path: list[int]) -> Ciphertext:
+        result = ct.clone()
+        for hop in path:
+            result = self.rotate_with_key(result, self.rotation_key(hop))
+        return result
+
+    def _apply_rotation_key_sequence(self, ct: Ciphertext, entries: list[tuple[int, RotationKey | None]], *,
+                                     use_hoisting: bool) -> list[Ciphertext]:
+        """Ordered direct keys (ckks_engine.py:3767-3820): hoisted when at
+        least two steps are non-zero, otherwise one rotation per key; step 0
+        returns a copy."""
+
+        nonzero = [k for s, k in entries if s != 0]
+        if not use_hoisting or len(nonzero) < 2:
+            return [ct.clone() if s == 0 else self.rotate_with_key(ct, k) for s, k in entries]
+        if self.fused_keyswitch and ct.batch_shape in ((), (1,)):
+            batch = self._rotate_hoisted_fused(ct, nonzero)
+            it = iter(range(len(nonzero)))
+            outs = []
+            for s, _ in entries:
+                if s == 0:
+                    outs.append(ct.clone())
+                    continue
+                e = next(it)
+                data = batch.data[:, e].reshape(ct.data.shape).clone()
+                outs.append(Ciphertext(data, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis))
+            return outs
+        hoisted = iter(self._rotate_hoisted_unfused(ct, nonzero))
+        return [ct.clone() if s == 0 else Ciphertext(next(hoisted), ct.level, ct.scale, "coefficient", "standard",
+                                                     ct.modulus_basis) for s, _ in entries]
+
+    def _rotate_hoisted_unfused(self, ct: Ciphertext, keys: list[RotationKey]) -> list[torch.Tensor]:
+        """The reference's hoisted op sequence (prepare_rotation_digits, then
+        per key the NTT-domain gather, MAC and ModDown) on the per-op kernels."""
+
         rt = self.runtime
         prepared = self.key_switcher.prepare_rotation_digits(ct.c1, ct.level)
         twice = rt.twice_modulus[rt.row_start(ct.level): self.config.num_q_primes]
         params = rt.params(ct.level)
         outs = []
         for key in keys:
-            if key.rotation_step == 0:
-                outs.append(ct.clone())
-                continue
             g = key.galois_element
             idx, sgn = rt.coefficient_galois(g)
             corr0, corr1 = self.key_switcher.apply_prepared_rotation(prepared, key, ct.level, g)
             c0 = K.apply_coefficient_galois_automorphism(ct.c0, idx, sgn, twice)
-            out = torch.stack([K.add_canonical(c0, corr0, params), corr1])
-            outs.append(Ciphertext(out, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis))
+            outs.append(torch.stack([K.add_canonical(c0, corr0, params), corr1]))
         return outs
+
+    def _rotate_hoisted_fused(self, ct: Ciphertext, keys: list[RotationKey]) -> Ciphertext:
+        data = self.key_switcher.key_switch_hoisted(ct.data[1], ct.data[0], keys, ct.level)
+        return Ciphertext(data, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
+
+    def rotate_hoisted(self, ct: Ciphertext, rotation_steps: list[int]) -> Ciphertext:
+        """Entry e of the result ([2, K, Q_level, N]) is the unbatched ``ct``
+        rotated by rotation_steps[e] with its direct key, all through one
+        shared decomposition and basis extension of c1 (the hoisted residues
+        of rotate_many_with_keys, returned as one batched ciphertext; no
+        reference counterpart).  Step 0 entries hold ``ct`` itself."""
+
+        self._assert_rotation_ciphertext(ct)
+        if ct.batch_shape not in ((), (1,)):
+            raise ValueError("rotate_hoisted takes one unbatched ciphertext")
+        steps = [RotationKey.canonical_step(s, self.config.N) for s in rotation_steps]
+        nonzero = [s for s in steps if s != 0]
+        lq, n = ct.data.shape[-2:]
+        out = torch.empty((2, len(steps), lq, n), dtype=torch.int64, device=self.device)
+        if nonzero:
+            keys = [self.rotation_key(s) for s in nonzero]
+            if self.fused_keyswitch:
+                rotated = self._rotate_hoisted_fused(ct, keys).data
+            else:
+                flat = ct.clone()
+                flat.data = ct.data.reshape(2, lq, n)
+                rotated = torch.stack(self._rotate_hoisted_unfused(flat, keys), dim=1)
+            pos = [e for e, s in enumerate(steps) if s != 0]
+            out[:, pos] = rotated
+        zero = [e for e, s in enumerate(steps) if s == 0]
+        if zero:
+            out[:, zero] = ct.data.reshape(2, 1, lq, n)
+        return Ciphertext(out, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
 
     # convenience: HMult as the reference's examples compose it (README.md:66-69)
     def hmult(self, a: Ciphertext, b: Ciphertext) -> Ciphertext:

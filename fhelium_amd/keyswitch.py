@@ -261,6 +261,91 @@ class HybridKeySwitcher:
         # same stream, which is ordered after these launches.
         return out.reshape(2, *batch_shape, out_rows, n)
 
+    def _galois_pairs(self, elements: list[int], dev: torch.device) -> torch.Tensor:
+        """Device [K, 2] table of (g, g^{-1} mod 2N) per entry, cached."""
+
+        cache = self.__dict__.setdefault("_galois_pair_cache", {})
+        tag = tuple(int(g) for g in elements)
+        hit = cache.get(tag)
+        if hit is None:
+            if len(cache) > 256:
+                cache.clear()
+            two_n = 2 * self.config.N
+            pairs = [(g % two_n, pow(g, -1, two_n)) for g in tag]
+            hit = torch.tensor(pairs, dtype=torch.int64).pin_memory().to(dev, non_blocking=True)
+            cache[tag] = hit
+        return hit
+
+    def key_switch_hoisted(self, component: torch.Tensor, base0: torch.Tensor,
+                           keys: list[KeySwitchKey], level: int) -> torch.Tensor:
+        """Hoisted rotations of one ciphertext (c0 = ``base0``, c1 =
+        ``component``, each [Q_level, N] or [1, Q_level, N], canonical
+        standard) by several rotation keys, in nine launches
+        (fh_keyswitch_hoisted): c1 is decomposed and extended once, and every
+        key's MAC reads that one intermediate through the NTT-domain Galois
+        map of the key's element, as prepare_rotation_digits +
+        _apply_prepared_rotation do (hybrid_keyswitch.py:373-504), with
+        sigma_g(c0) added canonically (ckks_engine.py:3767-3820).  Returns
+        [2, K, Q_level, N]: entry e is the rotation by keys[e]."""
+
+        from fhelium_amd import _native as nat
+
+        rt = self.rt
+        cfg = self.config
+        n = cfg.N
+        lq = cfg.num_q_primes - rt.row_start(level)
+        lqp = lq + cfg.num_p_primes
+        dev = component.device
+        _check_operand("component", component, 1, lq, n, dev)
+        _check_operand("base0", base0, 1, lq, n, dev)
+        if not keys:
+            raise ValueError("at least one rotation key is required")
+        k = len(keys)
+        data = [key.data for key in keys]
+        ref = data[0]
+        for t in data:
+            if t.shape != ref.shape or t.stride() != ref.stride():
+                raise ValueError("rotation keys must share shape and strides")
+        if ref.dim() != 4 or ref.size(1) != 2 or ref.size(2) != cfg.total_num_primes or ref.size(3) != n:
+            raise ValueError("key must be [digit, 2, QP, N]")
+        elements = [int(key.galois_element) for key in keys]
+        comp = component.reshape(1, lq, n)
+        c0 = base0.reshape(1, lq, n)
+        table = self.digit_table(level)
+        dnum = table.size(0)
+        ws_digits = torch.empty((k, lq, n), dtype=torch.int64, device=dev)
+        ws_inter = torch.empty((1, dnum, lqp, n), dtype=torch.int64, device=dev)
+        ws_acc = torch.empty((2, k, lqp, n), dtype=torch.int64, device=dev)
+        out = torch.empty((2, k, lq, n), dtype=torch.int64, device=dev)
+        bases = self._address_table(data, dev)
+        tables = [rt.coefficient_galois(g) for g in elements]
+        gal_idx = self._address_table([t[0] for t in tables], dev)
+        gal_sgn = self._address_table([t[1] for t in tables], dev)
+        pairs = self._galois_pairs(elements, dev)
+        start = rt.row_start(level)
+        small = [q < SMALL_PRIME_LIMIT for q in cfg.moduli[start: start + lqp]]
+        small_rows = sum(small)
+        if any(small[small_rows:]):
+            raise ValueError("fused key switch needs the binary64 rows before the integer rows")
+        nt = rt.ntt_tables
+        rec = self.recorder
+        if rec is not None:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
+        nat.call("fh_keyswitch_hoisted", nat.v3(comp), k, lq, lqp, start, cfg.logN, small_rows,
+                 table.data_ptr(), dnum, self.p_digit_table().data_ptr(), rt.p_inverse_q.data_ptr(),
+                 nat.t2(rt.rns_params), nt.rows.data_ptr(), nt.fwd.data_ptr(), nt.fwd_shoup.data_ptr(),
+                 nt.inv.data_ptr(), nt.inv_shoup.data_ptr(), bases.data_ptr(), ref.stride(0), ref.stride(1),
+                 ref.stride(2), pairs.data_ptr(), gal_idx.data_ptr(), gal_sgn.data_ptr(), ws_digits.data_ptr(),
+                 ws_inter.data_ptr(), ws_acc.data_ptr(), nat.v3(c0), nat.v3(out[0]), nat.v3(out[1]),
+                 rt.ntt._mode, nat.stream_ptr(dev))
+        if rec is not None:
+            ev1.record()
+            limb = n * 8
+            rec.append((ev0, ev1, limb * (2 * lq + k * 2 * lq + k * dnum * 2 * lqp)))
+        return out
+
     def _rescale_args(self, level: int, rescale: bool) -> tuple:
         """(q_level^{-1} mod q_i table, Shoup companions, floor(q_level / 2))
         for the rows above ``level``, cached on the device; nulls without a

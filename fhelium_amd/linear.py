@@ -4,13 +4,12 @@ The reference's packed matrix-vector benchmark
 (fhelium/benchmarks/standalone/packed_matrix_vector.py:120-170) computes
 y = sum_i d_i * rot(x, i) with hoisted rotations (rotate_many_with_keys),
 plaintext products and additions.  Here the K - 1 rotations of the one
-ciphertext run as batched fused key switches with one rotation key per batch
-entry (the ciphertext is broadcast over the batch with a zero stride, so it
-is not copied), the products as one batched plaintext multiply, and the sum
-as a pairwise tree of batched canonical additions; one inverse NTT and one
-rescale finish.  Rotations are computed directly (the automorphism applied
-before the key switch), so the ciphertexts differ from the hoisted path's
-but decrypt to the same values within the noise bound.
+ciphertext run as one hoisted fused key switch (``CkksEngine.rotate_hoisted``:
+c1 is decomposed and extended once, and every rotation key's MAC reads that
+intermediate through its NTT-domain Galois map), so the rotated ciphertexts
+carry exactly the reference's hoisted residues.  The products run as one
+batched plaintext multiply and the sum as a pairwise tree of batched
+canonical additions; one inverse NTT and one rescale finish.
 """
 
 from __future__ import annotations
@@ -45,10 +44,11 @@ def _sum_entries(engine, data: torch.Tensor, level: int) -> torch.Tensor:
     return data[:, 0]
 
 
-def diagonal_matvec(engine, ct: Ciphertext, diagonals: Plaintext, *, chunk: int = 32) -> Ciphertext:
+def diagonal_matvec(engine, ct: Ciphertext, diagonals: Plaintext, *, chunk: int = 128) -> Ciphertext:
     """y = sum_{i<K} d_i * rot(x, i), rescaled once: ``ct`` is one
     coefficient-domain ciphertext [2, Q_l, N]; ``diagonals`` the batched
-    NTT-domain plaintext from prepare_diagonals (K entries)."""
+    NTT-domain plaintext from prepare_diagonals (K entries).  Rotations run
+    ``chunk`` at a time, each chunk one hoisted key switch."""
 
     engine._assert_rotation_ciphertext(ct)
     if ct.batch_shape:
@@ -56,21 +56,12 @@ def diagonal_matvec(engine, ct: Ciphertext, diagonals: Plaintext, *, chunk: int 
     if len(diagonals.batch_shape) != 1 or diagonals.level != ct.level:
         raise ValueError("diagonals must be a [K, Q_l, N] plaintext at the ciphertext's level")
     k_total = diagonals.batch_shape[0]
-    lq, n = ct.data.shape[-2:]
     acc = None
     for start in range(0, k_total, chunk):
         steps = list(range(start, min(start + chunk, k_total)))
         b = len(steps)
         # entry i holds rot(x, steps[i]); step 0 is the ciphertext itself
-        rot_steps = [s for s in steps if s != 0]
-        parts = []
-        if 0 in steps:
-            parts.append(ct.data.unsqueeze(1))
-        if rot_steps:
-            broadcast = Ciphertext(ct.data.unsqueeze(1).expand(2, len(rot_steps), lq, n), ct.level, ct.scale)
-            parts.append(engine.rotate_by_steps(broadcast, rot_steps).data)
-        rotated = Ciphertext(torch.cat(parts, dim=1) if len(parts) > 1 else parts[0].contiguous(),
-                             ct.level, ct.scale)
+        rotated = engine.rotate_hoisted(ct, steps)
         pt = Plaintext(diagonals.data[start: start + b], diagonals.level, diagonals.scale, "ntt")
         prod = engine.multiply_plaintext(engine.coefficient_domain_to_ntt_domain(rotated), pt)
         part = _sum_entries(engine, prod.data, ct.level)

@@ -229,6 +229,37 @@ int fh_keyswitch_fused(fh_view3 component, int64_t batch, int64_t lq, int64_t lq
                        int64_t half_drop, const int64_t* seed0, const int64_t* seed1,
                        const int64_t* seed_scale, void* stream);
 
+/* Hoisted rotations of one ciphertext by K rotation keys
+ * (fhelium/engine/hybrid_keyswitch.py:373-504 prepare_rotation_digits +
+ * _apply_prepared_rotation, composed as ckks_engine.py:3767-3820 does for
+ * rotate_many_with_keys), bit for bit, in nine launches: c1 is decomposed
+ * and extended onto QP once (one entry), then for every entry e the tile
+ * pass + key MAC reads that intermediate through the NTT-domain Galois map
+ * of g_e (hybrid_keyswitch.py:506-528), the accumulators are inverse
+ * transformed, and ModDown adds sigma_{g_e}(c0) canonically.
+ *   component        c1, [1, lq, N] canonical standard rows at `level`
+ *   base0            c0, same shape (its batch stride is ignored)
+ *   key_bases        [entries] device addresses of the entries' keys
+ *                    [digit, 2, QP0, N] (strides given)
+ *   galois_elements  [entries, 2] int64 device: (g mod 2N, g^{-1} mod 2N)
+ *   galois_index_tables / galois_sign_tables: [entries] device addresses of
+ *                    each g's coefficient gather tables (int32 [N], int8 [N])
+ *   workspace_digits [entries, lq, N], workspace_inter [1, digit_count, lqp, N],
+ *                    workspace_acc [2, entries, lqp, N]
+ *   out0 / out1      [entries, lq, N]
+ * The other arguments are fh_keyswitch_fused's. */
+int fh_keyswitch_hoisted(fh_view3 component, int64_t entries, int64_t lq, int64_t lqp, int64_t level,
+                         int64_t log_n, int64_t small_rows, const int64_t* digit_table, int64_t digit_count,
+                         const int64_t* p_digit_table, const int64_t* p_inverse_q, fh_table2 rns_params,
+                         const void* ntt_rows, const int64_t* forward_twiddles,
+                         const int64_t* forward_twiddles_shoup, const int64_t* inverse_twiddles,
+                         const int64_t* inverse_twiddles_shoup, const int64_t* key_bases,
+                         int64_t key_stride_digit, int64_t key_stride_comp, int64_t key_stride_row,
+                         const int64_t* galois_elements, const int64_t* galois_index_tables,
+                         const int64_t* galois_sign_tables, int64_t* workspace_digits, int64_t* workspace_inter,
+                         int64_t* workspace_acc, fh_view3 base0, fh_view3 out0, fh_view3 out1, int32_t mode,
+                         void* stream);
+
 /* fh_keyswitch_fused without caller workspaces: no counterpart in the
  * reference.  It is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

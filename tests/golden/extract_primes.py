@@ -5,9 +5,10 @@ files (fhelium/config/resources/{scale,message}_primes_v1.safetensors, format
 validated by fhelium/config/_prime_catalog.py:93-167).  Presets select the
 first ``num_scale_primes`` scale primes and the first ``1 + num_p_primes``
 60-bit message primes (fhelium/config/ckks.py:379-535).  This script copies
-the sequences this backend supports into a JSON data file so the package never
-needs the reference at run time.  It is run by hand, once; the JSON is
-committed.
+the sequences this backend supports into the package's JSON data file
+(fhelium_amd/config/primes_v1.json) so the package never needs the reference
+at run time.  It is run by hand, once, in a tree where the reference is
+present; the JSON is committed.
 """
 
 from __future__ import annotations
@@ -19,7 +20,7 @@ import sys
 from safetensors import safe_open
 
 REFERENCE = pathlib.Path("/root/reference/fhelium/config/resources")
-OUT = pathlib.Path(__file__).with_name("primes_v1.json")
+OUT = pathlib.Path(__file__).resolve().parents[2] / "fhelium_amd" / "config" / "primes_v1.json"
 RING_DIMENSIONS = (1 << 14, 1 << 15, 1 << 16)
 
 

@@ -269,3 +269,31 @@ def test_rotation_composed_from_installed_keys():
     # a step 0 entry is left as it is
     out0 = eng.rotate_by_steps(a, [0, 4, 69])
     np.testing.assert_array_equal(_np(out0.data[:, 0]), _np(a.data[:, 0]))
+
+
+@pytest.mark.parametrize("level", [0, 6, 33])
+def test_fused_hoisted_rotations_bit_exact(small, level):
+    """Unbatched rotate_many_with_keys runs the fused hoisted key switch
+    (fh_keyswitch_hoisted): every result equals the oracle's restatement of
+    the reference's hoisted path (hybrid_keyswitch.py:373-504) bit for bit,
+    and the batched rotate_hoisted holds the same residues."""
+    eng, ref, _ = small
+    a, v = _fresh(eng, level, batch=(), seed=30 + level)
+    steps = [1, -7, 0, 513, 2, 1]
+    keys = [eng.rotation_key(s) for s in steps if s != 0]
+    got = eng.rotate_many_with_keys(a, keys)
+    prepared = ref.prepare_rotation_digits(_np(a.data)[1].reshape(1, a.limb_count, eng.config.N), level)
+    src = _np(a.data).reshape(2, 1, a.limb_count, eng.config.N)
+    for out, k in zip(got, keys):
+        want = ref.rotate_hoisted(src, prepared, _np(k.data), k.galois_element, level)
+        np.testing.assert_array_equal(_np(out.data).reshape(want.shape), want, err_msg=f"step {k.rotation_step}")
+    batch = eng.rotate_hoisted(a, steps)
+    it = iter(got)
+    for e, s in enumerate(steps):
+        want = _np(a.data) if s == 0 else _np(next(it).data)
+        np.testing.assert_array_equal(_np(batch.data[:, e]), want, err_msg=f"entry {e}")
+    many = eng.rotate_many_by_steps(a, steps)
+    for e, s in enumerate(steps):
+        np.testing.assert_array_equal(_np(many[e].data), _np(batch.data[:, e]), err_msg=f"by steps {e}")
+    err = np.abs(eng.decrypt(batch).real[3] - np.roll(v, 513)).max()
+    assert err < TOL_KEYSWITCH, err

@@ -167,3 +167,34 @@ def test_serving_full_size(full):
     for r in reqs:
         assert torch.equal(r.output.data, alone[r.rid]), f"request {r.rid} differs in the mix"
         assert r.output.level == levels[r.rid] + prog.depth
+
+
+@pytest.mark.parametrize("level", [0, 16, 32])
+def test_hoisted_rotations_full_size(full, level):
+    """The fused hoisted key switch (one decomposition of c1 shared by every
+    key, each key's MAC reading it through its NTT-domain Galois map) at
+    logN=16: sampled entries equal the oracle's hoisted path
+    (hybrid_keyswitch.py:373-504) bit for bit, every entry equals the
+    per-op hoisted sequence, and one entry decrypts within tolerance."""
+    eng, ref = full
+    x, ct = _encrypt(eng, level, 1, 700 + level)
+    ct.data = ct.data[:, 0]
+    print(f"hoisted rotations logN=16 level={level} keys={len(STEPS)}", flush=True)
+    out = eng.rotate_hoisted(ct, STEPS)
+    got = H.to_np(out.data)
+    src = H.to_np(ct.data)[:, None]
+    prepared = ref.prepare_rotation_digits(src[1], level)
+    for e in (0, 5):
+        key = eng.rotation_key(STEPS[e])
+        want = ref.rotate_hoisted(src, prepared, H.to_np(key.data), key.galois_element, level)
+        np.testing.assert_array_equal(got[:, e: e + 1], want, err_msg=f"entry {e} step {STEPS[e]}")
+    eng.fused_keyswitch = False
+    try:
+        unfused = eng.rotate_hoisted(ct, STEPS)
+    finally:
+        eng.fused_keyswitch = True
+    assert torch.equal(unfused.data, out.data)
+    one = out.clone()
+    one.data = out.data[:, 2]
+    err = np.abs(eng.decrypt(one).real - np.roll(x[0], STEPS[2])).max()
+    assert err < TOL_KEYSWITCH, err
