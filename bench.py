@@ -66,6 +66,9 @@ def parse_args():
     p.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     p.add_argument("--matvec-diagonals", type=int, default=128,
                    help="diagonals of the matrix-vector run (0 skips it)")
+    p.add_argument("--workload", default="batched", choices=["batched", "ntt", "matvec", "serving"],
+                   help="batched: the headline line (uneven HMult + rotation batch, with every side "
+                        "measurement); ntt / matvec / serving: only that configuration's measurement")
     p.add_argument("--calibration-copy", action="store_true",
                    help="run one 256 MiB device copy (torch.neg) before timing, as the byte "
                         "reference for hardware-counter traffic figures")
@@ -237,6 +240,30 @@ def cpu_baseline(eng, seconds_budget: float = 20.0):
             "torch_num_threads": torch.get_num_threads(), "cpu_model": _cpu_model()}
 
 
+def copy_ceiling(reps: int = 5) -> dict:
+    """HBM copy rate of the library's own 16-byte-vector copy kernel
+    (fh_stream_copy): 1 GiB (HBM-resident) and 64 MiB (fits the 256 MB
+    Infinity Cache), bytes read + written per second."""
+
+    from fhelium_amd import device as D
+
+    out = {}
+    for label, nbytes in (("hbm_1GiB", 1 << 30), ("cache_64MiB", 64 << 20)):
+        src = torch.randint(0, 1 << 60, (nbytes // 8,), dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        D.stream_copy(dst, src)
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+        for i in range(reps):
+            evs[2 * i].record()
+            D.stream_copy(dst, src)
+            evs[2 * i + 1].record()
+        torch.cuda.synchronize()
+        ms = float(np.median([evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(reps)]))
+        out[label] = {"bytes_moved": 2 * nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
+        del src, dst
+    return {"kernel": "fh_stream_copy (16-byte vector loads and stores)", **out}
+
+
 def ntt_measure(eng, reps: int = 5) -> dict:
     """Forward (to Montgomery) and inverse (to standard) NTT of the ring-kernel
     shape [64 ciphertexts, 2 polys, 35 limbs, 2^16], uniform residues,
@@ -354,6 +381,29 @@ def serving_run(eng, args) -> dict:
     }
 
 
+def side_workload(eng, args) -> int:
+    """One configuration on its own (profiling runs): prints its JSON line."""
+
+    if args.workload == "ntt":
+        copy = copy_ceiling()
+        ntt = ntt_measure(eng)
+        ntt["frac_of_copy_ceiling"] = ntt["achieved"] / copy["hbm_1GiB"]["GBps"]
+        result = {"metric": "forward + inverse NTT throughput, [64, 2, 35, 2^16] int64 residues",
+                  "value": ntt["achieved"], "unit": "GB/s (algorithmic)", "n_gpus": 1, "dtype": "int64",
+                  "roofline": ntt, "hbm_copy_ceiling": copy}
+    elif args.workload == "matvec":
+        mv = matvec_run(eng, args)
+        result = {"metric": "128-diagonal matrix-vector product (hoisted rotations), logN=16 L34",
+                  "value": mv["ms"], "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "int64",
+                  "matvec": mv}
+    else:
+        sv = serving_run(eng, args)
+        result = {"metric": "serving throughput at Poisson arrivals", "value": sv["throughput_per_s"],
+                  "unit": "requests/s", "n_gpus": 1, "dtype": "int64", "serving": sv}
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -374,6 +424,8 @@ def main() -> int:
     eng = CkksEngine(Preset.slots32768_scale40_levels34_int64, ntt_mode=args.ntt_mode, rng_seed=args.seed)
     eng.generate_secret_key()
     eng.generate_relinearization_key()
+    if args.workload != "batched":
+        return side_workload(eng, args)
     levels, steps, groups = build_workload(eng, args.count, args.seed)
     for i, s in enumerate(sorted(set(steps.tolist()))):
         eng.generate_rotation_key(int(s))
@@ -429,6 +481,8 @@ def main() -> int:
     batch_h = [a.elapsed_time(b) for a, b, _ in bh]
     batch_r = [a.elapsed_time(b) for a, b, _ in br]
     ntt = ntt_measure(eng)
+    copy = copy_ceiling()
+    ntt["frac_of_copy_ceiling"] = ntt["achieved"] / copy["hbm_1GiB"]["GBps"]
     # timed region only: the recorders start after the warmup steps
     ks_ms = sum(a.elapsed_time(b) for a, b, _ in ks_rec)
     ks_bytes = sum(nb for _, _, nb in ks_rec)
@@ -509,6 +563,7 @@ def main() -> int:
             "valu_instructions_per_algorithmic_byte": KS_VALU_INSTS_PER_ALG_BYTE,
         },
         "ntt_roofline": ntt,
+        "hbm_copy_ceiling": copy,
         "step_roofline": {
             "bound": "hbm", "algorithmic_bytes_per_step": step_bytes, "achieved": step_achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_achieved / HBM_PEAK_GBS,

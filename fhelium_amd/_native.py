@@ -17,7 +17,7 @@ import threading
 import torch
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "_lib" / "libfhelium_amd.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 
 class NativeExtensionError(RuntimeError):
@@ -89,6 +89,21 @@ SIGNATURES: dict[str, list] = {
     "fh_keyswitch_hoisted": [fh_view3, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, _i64, _ptr, _ptr, fh_table2,
                              _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _i64, _i64, _i64, _ptr, _ptr, _ptr, _ptr, _ptr,
                              _ptr, fh_view3, fh_view3, fh_view3, _i32, _ptr],
+    "fh_runtime_version": [_ptr],
+    "fh_stream_copy": [_ptr, _ptr, _i64, _ptr],
+    "fh_arena_create": [_i64, _ptr],
+    "fh_arena_alloc": [_ptr, _i64, _i64, _ptr],
+    "fh_arena_reset": [_ptr],
+    "fh_arena_stats": [_ptr, _ptr, _ptr, _ptr],
+    "fh_arena_destroy": [_ptr],
+    "fh_event_create": [_ptr],
+    "fh_event_record": [_ptr, _ptr],
+    "fh_event_elapsed_ms": [_ptr, _ptr, _ptr],
+    "fh_event_destroy": [_ptr],
+    "fh_graph_begin": [_ptr],
+    "fh_graph_end": [_ptr, _ptr],
+    "fh_graph_launch": [_ptr, _ptr],
+    "fh_graph_destroy": [_ptr],
     "fh_chacha20_keystream": [_ptr, _ptr, ctypes.c_uint32, _i64, _ptr, _ptr],
     "fh_sample_uniform_mod": [_ptr, _ptr, ctypes.c_uint32, _ptr, _i64, _i64, _i64, _ptr, _ptr],
     "fh_sample_ternary": [_ptr, _ptr, ctypes.c_uint32, _i64, _ptr, _ptr],
@@ -118,6 +133,19 @@ def load() -> ctypes.CDLL:
                 f"gfx950 library not built: {LIB_PATH} is missing; run "
                 "`python -m fhelium_amd.build`"
             )
+        from fhelium_amd import manifest
+
+        mpath = manifest.manifest_path(LIB_PATH.parent)
+        if not mpath.exists():
+            raise NativeExtensionError(
+                f"{LIB_PATH} has no build manifest ({mpath.name}); rebuild with `python -m fhelium_amd.build`")
+        try:
+            problems = manifest.compatibility_mismatches(manifest.read_manifest(mpath), lib_path=LIB_PATH,
+                                                         library_abi=ABI_VERSION)
+        except ValueError as exc:
+            raise NativeExtensionError(str(exc)) from exc
+        if problems:
+            raise NativeExtensionError(f"{LIB_PATH} does not match this process: " + "; ".join(problems))
         try:
             lib = ctypes.CDLL(str(LIB_PATH))
         except OSError as exc:  # pragma: no cover - depends on the host

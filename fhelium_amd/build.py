@@ -2,6 +2,9 @@
 
 The library is a plain C ABI (declared in ``include/fhelium_amd.h``) with no
 PyTorch types in its signatures; the Python package loads it with ctypes.
+Next to it the build writes ``_build_manifest.<soabi>.json`` (fhelium_amd/
+manifest.py), which the loader checks.  ``CMakeLists.txt`` in this package
+builds the same library with CMake's HIP language support.
 Run as ``python -m fhelium_amd.build`` or through ``__graft_entry__.build()``.
 """
 
@@ -70,6 +73,10 @@ def build(verbose: bool = False) -> pathlib.Path:
     if verbose and (proc.stdout or proc.stderr):
         sys.stderr.write(proc.stdout + proc.stderr)
     os.replace(tmp, LIB_PATH)
+    from fhelium_amd import _native, manifest
+
+    manifest.write_manifest(LIB_PATH, hip_version=manifest.hipcc_version(hipcc()), offload_arch=OFFLOAD_ARCH,
+                            library_abi=_native.ABI_VERSION)
     return LIB_PATH
 
 

@@ -607,214 +607,52 @@ __device__ __forceinline__ int hoist_dst_slot(int k, int h, uint32_t ginv) {
   return static_cast<int>(brev_bits(((brev_bits(static_cast<uint32_t>(k), 3) - static_cast<uint32_t>(h)) * ginv) & 7u, 3));
 }
 
-// kHoist: the block's tiles are source tiles of a hoisted rotation's
-// NTT-domain Galois map (see hoist_slot); otherwise source and destination
-// tiles coincide.
-template <bool kHoist, class P>
-__device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g, int64_t j,
-                                          int64_t b, int64_t* lds) {
-  using V = typename P::V;
-  constexpr bool kF64 = std::is_same<V, double>::value;
+// Destination of a MAC block's sums.  For a hoisted rotation the block's
+// tiles r are source tiles of the NTT-domain Galois map: the accumulators
+// belong to tile r_dst, and this thread's source slot k to destination
+// element dst_base + slot k (3 bits per slot in slot_map).  Otherwise
+// r_dst = r and slot k is element 8t + k.
+struct MacDest {
+  int r_dst;
+  int dst_base;
+  uint32_t slot_map;
+  __device__ __forceinline__ int slot(int k) const {
+    return dst_base + static_cast<int>((slot_map >> (3 * k)) & 7u);
+  }
+};
+
+__device__ __forceinline__ MacDest hoisted_dest(const KsArgs& a, int r, int t, int64_t b) {
+  const uint32_t hg = static_cast<uint32_t>(a.hoist_galois[2 * b]);
+  const uint32_t hginv = static_cast<uint32_t>(a.hoist_galois[2 * b + 1]);
+  const HoistSlot hs = hoist_slot(r, t, hg, hginv, a.log_n);
+  MacDest d{hs.dst_tile, hs.dst_base, 0u};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) d.slot_map |= static_cast<uint32_t>(hoist_dst_slot(k, hs.h, hginv)) << (3 * k);
+  return d;
+}
+
+// The accumulators of tile dst.r_dst (thread t holding elements 8t..8t+7):
+// the first 8 inverse stages, which act inside the tile (fast arithmetic, as
+// the rest of the inverse), then the working-format store the inverse column
+// pass reads.  With kHoist the sums arrive in source order and first move to
+// the destination layout through the exchange buffer.
+template <bool kHoist, class Acc>
+__device__ __forceinline__ void mac_finish(const KsArgs& a, int64_t j, int64_t b, int t, const MacDest& dst,
+                                           int64_t* sh, int64_t* tww, int64_t* twp, Acc (&acc0)[8],
+                                           Acc (&acc1)[8]) {
+  constexpr bool kF64 = std::is_same<Acc, double>::value;
   const int64_t n = int64_t{1} << a.log_n;
   const int64_t grow = a.level + j;
   const NttRow& row = a.rows[grow];
-  const int m = threadIdx.x / kMacThreads;
-  const int t = threadIdx.x % kMacThreads;
-  const int r = static_cast<int>(g) * kMacTiles + m;
-  const int s0 = a.log_n - kTileLog;
-  int64_t* sh = lds + m * kTilePitch;
-  int64_t* tww = lds + kMacExchange + m * kTile;
-  int64_t* twp = tww + kMacTiles * kTile;  // integer rows only
-  int64_t* key_digit = lds + kMacExchange + kMacTiles * kTile * (kF64 ? 1 : 2);
-  // Destination tile of the accumulators and, packed 3 bits per slot, the
-  // destination slot of each of this thread's 8 source slots.
-  int r_dst = r;
-  int dst_base = 8 * t;
-  uint32_t slot_map = 0xFAC688u;  // slots 0..7 in order
-  const int64_t b_in = kHoist ? 0 : b;  // hoisted entries share one intermediate
+  const int r_dst = dst.r_dst;
   if constexpr (kHoist) {
-    const uint32_t hg = static_cast<uint32_t>(a.hoist_galois[2 * b]);
-    const uint32_t hginv = static_cast<uint32_t>(a.hoist_galois[2 * b + 1]);
-    const HoistSlot hs = hoist_slot(r, t, hg, hginv, a.log_n);
-    r_dst = hs.dst_tile;
-    dst_base = hs.dst_base;
-    slot_map = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) slot_map |= static_cast<uint32_t>(hoist_dst_slot(k, hs.h, hginv)) << (3 * k);
-  }
-  auto key_slot = [&](int k) {
-    if constexpr (kHoist) return dst_base + static_cast<int>((slot_map >> (3 * k)) & 7u);
-    return mac_e3(t, k);
-  };
-  {
-    const int64_t* w_row = a.fwd_w + grow * n;
-    const int64_t* wp_row = a.fwd_wp + grow * n;
-    for (int idx = t; idx < kTile - 1; idx += kMacThreads) {
-      const int st = 31 - __builtin_clz(idx + 1);
-      const int64_t gi = (int64_t{1} << (s0 + st)) + (static_cast<int64_t>(r) << st) + (idx + 1 - (1 << st));
-      tww[idx] = w_row[gi];
-      if constexpr (!kF64) twp[idx] = wp_row[gi];
-    }
-    if (threadIdx.x < a.dnum) key_digit[threadIdx.x] = a.dig[threadIdx.x].key_digit;
-  }
-  __syncthreads();
-  auto tw_at = [&](int st, int e) {
-    const int slot = mac_tw_slot(st, e);
-    return p.tw_make(tww[slot], kF64 ? 0 : twp[slot]);
-  };
-  Global<int64_t>* key = global(a.key_bases ? reinterpret_cast<const int64_t*>(a.key_bases[b]) : a.key);
-  const int64_t tile_off = static_cast<int64_t>(r) * kTile;       // source tile
-  const int64_t dst_off = static_cast<int64_t>(r_dst) * kTile;    // key and accumulators
-  double facc0[8], facc1[8];
-  uint64_t uacc0[8], uacc1[8];
-  WideAcc wacc0[8], wacc1[8];  // integer rows: exact sums, one REDC after the digits
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    facc0[k] = facc1[k] = 0.0;
-    wacc0[k] = wacc1[k] = WideAcc{0, 0, 0};
-  }
-  const int64_t* tile = a.inter + ((b_in * a.dnum) * a.lqp + j) * n + tile_off;
-  const int64_t digit_stride = a.lqp * n;
-#pragma nounroll
-  for (int64_t di = 0; di < a.dnum; ++di, tile += digit_stride) {
-    Global<int64_t>* k0 = key + key_digit[di] * a.key_sd + grow * a.key_sr + dst_off;
-    Global<int64_t>* k1 = k0 + a.key_sc;
-    V v[8];
-    int64_t w0[8], w1[8];
-    if constexpr (std::is_same<P, F64Fast>::value) {
-      // Eight consecutive coefficients 8t..8t+7 of the tile are contiguous
-      // in the packed block layout (one 32-byte and one 16-byte load); an
-      // LDS pass moves them to the round-1 ownership.
-      const PackedInter pk = packed_slab(a.inter + (b_in * a.dnum + di) * a.lqp * n, a.small_rows, n);
-      const uint32_t idx = static_cast<uint32_t>(j * n + packed_index(r, 8 * t));
-      const uint4 l0 = *reinterpret_cast<const uint4*>(pk.lo + idx);
-      const uint4 l1 = *reinterpret_cast<const uint4*>(pk.lo + idx + 4);
-      const uint4 hh = *reinterpret_cast<const uint4*>(pk.hi + idx);
-      const uint32_t lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
-      const uint32_t hi2[4] = {hh.x, hh.y, hh.z, hh.w};
-      tile_sync();  // the previous digit's last reads of the exchange buffer are done
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const uint64_t h = (hi2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
-        sh[tile_pad(mac_e3(t, k))] = __double_as_longlong(u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32)));
-      }
-      tile_sync();
-#pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
-    } else {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] = p.unmid(tile[mac_e1(t, k)]);
-    }
-    // Binary64 rows issue the key loads with the tile loads; the integer
-    // rows (whose products need many more registers) load them at the MAC.
-    if constexpr (kF64) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        w0[k] = k0[key_slot(k)];
-        w1[k] = k1[key_slot(k)];
-      }
-    }
-    // Round 1: tile stages 0..2 pair element bits 7..5 = bits 2..0 of k.
-#pragma unroll
-    for (int st = 0; st < 3; ++st) {
-      const int half = 4 >> st;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        if (k & half) continue;
-        p.fwd(v[k], v[k + half], tw_at(st, mac_e1(t, k)));
-      }
-    }
-    // The previous digit's last reads of this buffer must be done.
-    tile_sync();
-#pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
-    tile_sync();
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
-    // Round 2: tile stages 3..5 pair element bits 4..2 = bits 2..0 of k.
-#pragma unroll
-    for (int st = 3; st < 6; ++st) {
-      const int half = 1 << (5 - st);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        if (k & half) continue;
-        p.fwd(v[k], v[k + half], tw_at(st, mac_e2(t, k)));
-      }
-    }
-    // Each thread rewrites the words it just read: no barrier needed.
-#pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
-    tile_sync();
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
-    // Round 3: tile stages 6..7 pair element bits 1..0 = bits 1..0 of k.
-#pragma unroll
-    for (int st = 6; st < 8; ++st) {
-      const int half = 1 << (7 - st);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        if (k & half) continue;
-        p.fwd(v[k], v[k + half], tw_at(st, mac_e3(t, k)));
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if constexpr (kF64) {
-        mac_f64(facc0[k], v[k], w0[k], row.qd, row.qinv);
-        mac_f64(facc1[k], v[k], w1[k], row.qd, row.qinv);
-      } else {
-        w0[k] = k0[key_slot(k)];
-        w1[k] = k1[key_slot(k)];
-        wide_mac(wacc0[k], v[k], static_cast<uint64_t>(w0[k]));
-        wide_mac(wacc1[k], v[k], static_cast<uint64_t>(w1[k]));
-      }
-    }
-  }
-  // Relinearization seeds: P d0 and P d1 (NTT domain, in the accumulators'
-  // scaling) join the Q rows' sums, so that ModDown's exact floor division
-  // returns floor(X / P) + d directly.  Thread t holds coefficients 8t + k,
-  // stored at tile positions t + 32k (the HMult front's seed layout).
-  if (!kHoist && a.seed0 != nullptr && j < a.lq) {
-    const int64_t off = (b * a.lq + j) * n + tile_off + t;
-    if constexpr (kF64) {
-      const double sc = static_cast<double>(a.seed_scale[grow]);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        facc0[k] += f64_mulmod_signed(sc, __longlong_as_double(a.seed0[off + 32 * k]), row.qd, row.qinv);
-        facc1[k] += f64_mulmod_signed(sc, __longlong_as_double(a.seed1[off + 32 * k]), row.qd, row.qinv);
-      }
-    } else {
-      const uint64_t sc = static_cast<uint64_t>(a.seed_scale[grow]);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        wide_mac(wacc0[k], static_cast<uint64_t>(a.seed0[off + 32 * k]), sc);
-        wide_mac(wacc1[k], static_cast<uint64_t>(a.seed1[off + 32 * k]), sc);
-      }
-    }
-  }
-  if constexpr (!kF64) {
-    // Each sum holds at most 16 products y w with y < 4q (forward outputs)
-    // and w < 2q (key words), so it is below 128 q^2 < 2^127 and its REDC
-    // value r = sum 2^-64 mod q is below 128 q^2 / 2^64 + q < 9q (q < 2^60).
-    // The binary64 rows' REDC radix is 2^62: 4 r is the same residue as
-    // sum 2^-62.  Result in [0, 2q), the inverse butterflies' input range.
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      uacc0[k] = wide_redc62(wacc0[k], row.q, row.neg_qinv64);
-      uacc1[k] = wide_redc62(wacc1[k], row.q, row.neg_qinv64);
-    }
-  }
-  // Hoisted: source slot k of this thread accumulated destination element
-  // dst_base + slot k; move the sums to the destination layout (thread t
-  // holding elements 8t..8t+7 of tile r_dst) through the exchange buffer.
-  if constexpr (kHoist) {
-    auto to_dst = [&](auto (&acc)[8]) {
+    auto to_dst = [&](Acc (&acc)[8]) {
       tile_sync();  // earlier readers of the exchange buffer are done
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         int64_t bits;
         __builtin_memcpy(&bits, &acc[k], sizeof(bits));
-        sh[tile_pad(key_slot(k))] = bits;
+        sh[tile_pad(dst.slot(k))] = bits;
       }
       tile_sync();
 #pragma unroll
@@ -823,17 +661,9 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
         __builtin_memcpy(&acc[k], &bits, sizeof(bits));
       }
     };
-    if constexpr (kF64) {
-      to_dst(facc0);
-      to_dst(facc1);
-    } else {
-      to_dst(uacc0);
-      to_dst(uacc1);
-    }
+    to_dst(acc0);
+    to_dst(acc1);
   }
-  // The first 8 stages of the accumulators' inverse NTT act inside this
-  // tile: run them here (fast arithmetic, as the rest of the inverse) and
-  // write the working-format result; only the column pass remains.
   tile_sync();
   {
     const int64_t* w_row = a.inv_w + grow * n;
@@ -904,18 +734,198 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     const PackedInter pk0 = packed_slab(a.acc + (0 * a.batch + b) * a.lqp * n, a.small_rows, n);
     const PackedInter pk1 = packed_slab(a.acc + (1 * a.batch + b) * a.lqp * n, a.small_rows, n);
     const int64_t row_off = j * n;
-    inverse_tile(pi, facc0, [&](int e, double x) {
+    inverse_tile(pi, acc0, [&](int e, double x) {
       packed_store(pk0, row_off + packed_index(r_dst, e), f64_to_u64(pi.canon(x)));
     });
-    inverse_tile(pi, facc1, [&](int e, double x) {
+    inverse_tile(pi, acc1, [&](int e, double x) {
       packed_store(pk1, row_off + packed_index(r_dst, e), f64_to_u64(pi.canon(x)));
     });
   } else {
     const U64Fast pi(row);
-    int64_t* out0 = a.acc + ((0 * a.batch + b) * a.lqp + j) * n + dst_off;
-    int64_t* out1 = a.acc + ((1 * a.batch + b) * a.lqp + j) * n + dst_off;
-    inverse_tile(pi, uacc0, [&](int e, uint64_t x) { out0[e] = static_cast<int64_t>(x); });
-    inverse_tile(pi, uacc1, [&](int e, uint64_t x) { out1[e] = static_cast<int64_t>(x); });
+    int64_t* out0 = a.acc + ((0 * a.batch + b) * a.lqp + j) * n + static_cast<int64_t>(r_dst) * kTile;
+    int64_t* out1 = a.acc + ((1 * a.batch + b) * a.lqp + j) * n + static_cast<int64_t>(r_dst) * kTile;
+    inverse_tile(pi, acc0, [&](int e, uint64_t x) { out0[e] = static_cast<int64_t>(x); });
+    inverse_tile(pi, acc1, [&](int e, uint64_t x) { out1[e] = static_cast<int64_t>(x); });
+  }
+}
+
+template <class P>
+__device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g, int64_t j,
+                                          int64_t b, int64_t* lds) {
+  using V = typename P::V;
+  constexpr bool kF64 = std::is_same<V, double>::value;
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t grow = a.level + j;
+  const NttRow& row = a.rows[grow];
+  const int m = threadIdx.x / kMacThreads;
+  const int t = threadIdx.x % kMacThreads;
+  const int r = static_cast<int>(g) * kMacTiles + m;
+  const int s0 = a.log_n - kTileLog;
+  int64_t* sh = lds + m * kTilePitch;
+  int64_t* tww = lds + kMacExchange + m * kTile;
+  int64_t* twp = tww + kMacTiles * kTile;  // integer rows only
+  int64_t* key_digit = lds + kMacExchange + kMacTiles * kTile * (kF64 ? 1 : 2);
+  {
+    const int64_t* w_row = a.fwd_w + grow * n;
+    const int64_t* wp_row = a.fwd_wp + grow * n;
+    for (int idx = t; idx < kTile - 1; idx += kMacThreads) {
+      const int st = 31 - __builtin_clz(idx + 1);
+      const int64_t gi = (int64_t{1} << (s0 + st)) + (static_cast<int64_t>(r) << st) + (idx + 1 - (1 << st));
+      tww[idx] = w_row[gi];
+      if constexpr (!kF64) twp[idx] = wp_row[gi];
+    }
+    if (threadIdx.x < a.dnum) key_digit[threadIdx.x] = a.dig[threadIdx.x].key_digit;
+  }
+  __syncthreads();
+  auto tw_at = [&](int st, int e) {
+    const int slot = mac_tw_slot(st, e);
+    return p.tw_make(tww[slot], kF64 ? 0 : twp[slot]);
+  };
+  Global<int64_t>* key = global(a.key_bases ? reinterpret_cast<const int64_t*>(a.key_bases[b]) : a.key);
+  const int64_t tile_off = static_cast<int64_t>(r) * kTile;
+  double facc0[8], facc1[8];
+  uint64_t uacc0[8], uacc1[8];
+  WideAcc wacc0[8], wacc1[8];  // integer rows: exact sums, one REDC after the digits
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    facc0[k] = facc1[k] = 0.0;
+    wacc0[k] = wacc1[k] = WideAcc{0, 0, 0};
+  }
+  const int64_t* tile = a.inter + ((b * a.dnum) * a.lqp + j) * n + tile_off;
+  const int64_t digit_stride = a.lqp * n;
+#pragma nounroll
+  for (int64_t di = 0; di < a.dnum; ++di, tile += digit_stride) {
+    Global<int64_t>* k0 = key + key_digit[di] * a.key_sd + grow * a.key_sr + tile_off;
+    Global<int64_t>* k1 = k0 + a.key_sc;
+    V v[8];
+    int64_t w0[8], w1[8];
+    if constexpr (std::is_same<P, F64Fast>::value) {
+      // Eight consecutive coefficients 8t..8t+7 of the tile are contiguous
+      // in the packed block layout (one 32-byte and one 16-byte load); an
+      // LDS pass moves them to the round-1 ownership.
+      const PackedInter pk = packed_slab(a.inter + (b * a.dnum + di) * a.lqp * n, a.small_rows, n);
+      const uint32_t idx = static_cast<uint32_t>(j * n + packed_index(r, 8 * t));
+      const uint4 l0 = *reinterpret_cast<const uint4*>(pk.lo + idx);
+      const uint4 l1 = *reinterpret_cast<const uint4*>(pk.lo + idx + 4);
+      const uint4 hh = *reinterpret_cast<const uint4*>(pk.hi + idx);
+      const uint32_t lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+      const uint32_t hi2[4] = {hh.x, hh.y, hh.z, hh.w};
+      tile_sync();  // the previous digit's last reads of the exchange buffer are done
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint64_t h = (hi2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+        sh[tile_pad(mac_e3(t, k))] = __double_as_longlong(u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32)));
+      }
+      tile_sync();
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = p.unmid(tile[mac_e1(t, k)]);
+    }
+    // Binary64 rows issue the key loads with the tile loads; the integer
+    // rows (whose products need many more registers) load them at the MAC.
+    if constexpr (kF64) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        w0[k] = k0[mac_e3(t, k)];
+        w1[k] = k1[mac_e3(t, k)];
+      }
+    }
+    // Round 1: tile stages 0..2 pair element bits 7..5 = bits 2..0 of k.
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+      const int half = 4 >> st;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half], tw_at(st, mac_e1(t, k)));
+      }
+    }
+    // The previous digit's last reads of this buffer must be done.
+    tile_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
+    tile_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
+    // Round 2: tile stages 3..5 pair element bits 4..2 = bits 2..0 of k.
+#pragma unroll
+    for (int st = 3; st < 6; ++st) {
+      const int half = 1 << (5 - st);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half], tw_at(st, mac_e2(t, k)));
+      }
+    }
+    // Each thread rewrites the words it just read: no barrier needed.
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
+    tile_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
+    // Round 3: tile stages 6..7 pair element bits 1..0 = bits 1..0 of k.
+#pragma unroll
+    for (int st = 6; st < 8; ++st) {
+      const int half = 1 << (7 - st);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k & half) continue;
+        p.fwd(v[k], v[k + half], tw_at(st, mac_e3(t, k)));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if constexpr (kF64) {
+        mac_f64(facc0[k], v[k], w0[k], row.qd, row.qinv);
+        mac_f64(facc1[k], v[k], w1[k], row.qd, row.qinv);
+      } else {
+        w0[k] = k0[mac_e3(t, k)];
+        w1[k] = k1[mac_e3(t, k)];
+        wide_mac(wacc0[k], v[k], static_cast<uint64_t>(w0[k]));
+        wide_mac(wacc1[k], v[k], static_cast<uint64_t>(w1[k]));
+      }
+    }
+  }
+  // Relinearization seeds: P d0 and P d1 (NTT domain, in the accumulators'
+  // scaling) join the Q rows' sums, so that ModDown's exact floor division
+  // returns floor(X / P) + d directly.  Thread t holds coefficients 8t + k,
+  // stored at tile positions t + 32k (the HMult front's seed layout).
+  if (a.seed0 != nullptr && j < a.lq) {
+    const int64_t off = (b * a.lq + j) * n + tile_off + t;
+    if constexpr (kF64) {
+      const double sc = static_cast<double>(a.seed_scale[grow]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        facc0[k] += f64_mulmod_signed(sc, __longlong_as_double(a.seed0[off + 32 * k]), row.qd, row.qinv);
+        facc1[k] += f64_mulmod_signed(sc, __longlong_as_double(a.seed1[off + 32 * k]), row.qd, row.qinv);
+      }
+    } else {
+      const uint64_t sc = static_cast<uint64_t>(a.seed_scale[grow]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        wide_mac(wacc0[k], static_cast<uint64_t>(a.seed0[off + 32 * k]), sc);
+        wide_mac(wacc1[k], static_cast<uint64_t>(a.seed1[off + 32 * k]), sc);
+      }
+    }
+  }
+  if constexpr (!kF64) {
+    // Each sum holds at most 16 products y w with y < 4q (forward outputs)
+    // and w < 2q (key words), so it is below 128 q^2 < 2^127 and its REDC
+    // value r = sum 2^-64 mod q is below 128 q^2 / 2^64 + q < 9q (q < 2^60).
+    // The binary64 rows' REDC radix is 2^62: 4 r is the same residue as
+    // sum 2^-62.  Result in [0, 2q), the inverse butterflies' input range.
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      uacc0[k] = wide_redc62(wacc0[k], row.q, row.neg_qinv64);
+      uacc1[k] = wide_redc62(wacc1[k], row.q, row.neg_qinv64);
+    }
+  }
+  if constexpr (kF64) {
+    mac_finish<false>(a, j, b, t, MacDest{r, 8 * t, 0u}, sh, tww, twp, facc0, facc1);
+  } else {
+    mac_finish<false>(a, j, b, t, MacDest{r, 8 * t, 0u}, sh, tww, twp, uacc0, uacc1);
   }
 }
 
@@ -927,7 +937,7 @@ constexpr int kMacTilesPerBlock = kMacTiles;
 
 // One launch per arithmetic path (binary64 rows, then the integer rows) so
 // each kernel is register-allocated for its own path only.
-template <int MODE, bool kSmall, bool kHoist>
+template <int MODE, bool kSmall>
 // Register budgets: 3 waves per SIMD for the binary64 rows (no spills at 156
 // VGPRs), 2+ for the integer rows.
 __global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 3 : 2))) void k_ks_tiles_mac(KsArgs a, int64_t groups, int64_t j_begin,
@@ -942,12 +952,271 @@ __global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmal
   const NttRow& row = a.rows[a.level + j];
   if constexpr (kSmall) {
     if constexpr (MODE == 1) {
-      tiles_mac<kHoist>(F64Fast(row), a, g, j, b, lds);
+      tiles_mac(F64Fast(row), a, g, j, b, lds);
     } else {
-      tiles_mac<kHoist>(F64Exact(row), a, g, j, b, lds);
+      tiles_mac(F64Exact(row), a, g, j, b, lds);
     }
   } else {
-    tiles_mac<kHoist>(U64Fast(row), a, g, j, b, lds);
+    tiles_mac(U64Fast(row), a, g, j, b, lds);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hoisted rotations.  Every rotation of the one input reads the same
+// extended digits, so their forward tile pass runs once (k_ks_tiles_fwd,
+// in place), and the per-rotation MAC (k_ks_mac_hoisted) only loads the
+// transformed digits through the rotation's NTT-domain Galois map and
+// multiplies them by its key.
+// ---------------------------------------------------------------------------
+
+// Forward tile pass of tile r of row j of digit di of the one input entry,
+// stored back in place fully transformed: canonical residues in the packed
+// planes (fast binary64 rows), binary64 bits below 2q (exact binary64 rows)
+// or u64 below 4q (integer rows).  Each tile's threads load all of its
+// elements before any of them stores, and tiles are disjoint.
+template <class P>
+__device__ __forceinline__ void tiles_forward_store(const P& p, const KsArgs& a, int64_t g, int64_t j,
+                                                    int64_t di, int64_t* lds) {
+  using V = typename P::V;
+  constexpr bool kF64 = std::is_same<V, double>::value;
+  constexpr bool kPacked = std::is_same<P, F64Fast>::value;
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t grow = a.level + j;
+  const int m = threadIdx.x / kMacThreads;
+  const int t = threadIdx.x % kMacThreads;
+  const int r = static_cast<int>(g) * kMacTiles + m;
+  const int s0 = a.log_n - kTileLog;
+  int64_t* sh = lds + m * kTilePitch;
+  int64_t* tww = lds + kMacExchange + m * kTile;
+  int64_t* twp = tww + kMacTiles * kTile;  // integer rows only
+  {
+    const int64_t* w_row = a.fwd_w + grow * n;
+    const int64_t* wp_row = a.fwd_wp + grow * n;
+    for (int idx = t; idx < kTile - 1; idx += kMacThreads) {
+      const int st = 31 - __builtin_clz(idx + 1);
+      const int64_t gi = (int64_t{1} << (s0 + st)) + (static_cast<int64_t>(r) << st) + (idx + 1 - (1 << st));
+      tww[idx] = w_row[gi];
+      if constexpr (!kF64) twp[idx] = wp_row[gi];
+    }
+  }
+  tile_sync();
+  auto tw_at = [&](int st, int e) {
+    const int slot = mac_tw_slot(st, e);
+    return p.tw_make(tww[slot], kF64 ? 0 : twp[slot]);
+  };
+  int64_t* slab = a.inter + di * a.lqp * n;
+  int64_t* tile = slab + j * n + static_cast<int64_t>(r) * kTile;
+  V v[8];
+  const PackedInter pk = packed_slab(slab, a.small_rows, n);
+  const uint32_t pidx = static_cast<uint32_t>(j * n + packed_index(r, 8 * t));
+  if constexpr (kPacked) {
+    const uint4 l0 = *reinterpret_cast<const uint4*>(pk.lo + pidx);
+    const uint4 l1 = *reinterpret_cast<const uint4*>(pk.lo + pidx + 4);
+    const uint4 hh = *reinterpret_cast<const uint4*>(pk.hi + pidx);
+    const uint32_t lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+    const uint32_t hi2[4] = {hh.x, hh.y, hh.z, hh.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint64_t h = (hi2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+      sh[tile_pad(mac_e3(t, k))] = __double_as_longlong(u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32)));
+    }
+    tile_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(tile[mac_e1(t, k)]);
+  }
+#pragma unroll
+  for (int st = 0; st < 3; ++st) {
+    const int half = 4 >> st;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k & half) continue;
+      p.fwd(v[k], v[k + half], tw_at(st, mac_e1(t, k)));
+    }
+  }
+  tile_sync();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
+  tile_sync();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
+#pragma unroll
+  for (int st = 3; st < 6; ++st) {
+    const int half = 1 << (5 - st);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k & half) continue;
+      p.fwd(v[k], v[k + half], tw_at(st, mac_e2(t, k)));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
+  tile_sync();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
+#pragma unroll
+  for (int st = 6; st < 8; ++st) {
+    const int half = 1 << (7 - st);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k & half) continue;
+      p.fwd(v[k], v[k + half], tw_at(st, mac_e3(t, k)));
+    }
+  }
+  if constexpr (kPacked) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) packed_store(pk, pidx + k, f64_to_u64(p.canon(v[k])));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) tile[mac_e3(t, k)] = p.mid(v[k]);
+  }
+}
+
+template <int MODE, bool kSmall>
+__global__ __launch_bounds__(kMacBlock) void k_ks_tiles_fwd(KsArgs a, int64_t groups, int64_t j_begin,
+                                                            int64_t j_count) {
+  __shared__ int64_t lds[kMacExchange + kMacTiles * kTile * (kSmall ? 1 : 2)];
+  const uint32_t dnum = static_cast<uint32_t>(a.dnum);
+  const uint32_t jc = static_cast<uint32_t>(j_count);
+  const uint32_t w = xcd_remap(blockIdx.x, static_cast<uint32_t>(groups) * jc * dnum);
+  const int64_t di = w % dnum;
+  const int64_t j = j_begin + (w / dnum) % jc;
+  const int64_t g = w / (dnum * jc);
+  const NttRow& row = a.rows[a.level + j];
+  if constexpr (kSmall) {
+    if constexpr (MODE == 1) {
+      tiles_forward_store(F64Fast(row), a, g, j, di, lds);
+    } else {
+      tiles_forward_store(F64Exact(row), a, g, j, di, lds);
+    }
+  } else {
+    tiles_forward_store(U64Fast(row), a, g, j, di, lds);
+  }
+}
+
+// One rotation (entry b) of the hoisted MAC over the transformed digits.
+// The block is source-stationary: it reads source tiles r of every digit
+// (the same tiles for every entry, so consecutive entries on an XCD find
+// them in its L2) and accumulates, in source order, the products with the
+// key words at the destination elements; mac_finish moves the sums to
+// destination tile order.
+template <class P>
+__device__ __forceinline__ void tiles_mac_hoisted(const P& p, const KsArgs& a, int64_t g, int64_t j,
+                                                  int64_t b, int64_t* lds) {
+  using V = typename P::V;
+  constexpr bool kF64 = std::is_same<V, double>::value;
+  constexpr bool kPacked = std::is_same<P, F64Fast>::value;
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t grow = a.level + j;
+  const NttRow& row = a.rows[grow];
+  const int m = threadIdx.x / kMacThreads;
+  const int t = threadIdx.x % kMacThreads;
+  const int r = static_cast<int>(g) * kMacTiles + m;
+  int64_t* sh = lds + m * kTilePitch;
+  int64_t* tww = lds + kMacExchange + m * kTile;
+  int64_t* twp = tww + kMacTiles * kTile;  // integer rows only
+  int64_t* key_digit = lds + kMacExchange + kMacTiles * kTile * (kF64 ? 1 : 2);
+  if (threadIdx.x < a.dnum) key_digit[threadIdx.x] = a.dig[threadIdx.x].key_digit;
+  __syncthreads();
+  const MacDest dst = hoisted_dest(a, r, t, b);
+  Global<int64_t>* key = global(a.key_bases ? reinterpret_cast<const int64_t*>(a.key_bases[b]) : a.key);
+  const int64_t dst_off = static_cast<int64_t>(dst.r_dst) * kTile;
+  double facc0[8], facc1[8];
+  uint64_t uacc0[8], uacc1[8];
+  WideAcc wacc0[8], wacc1[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    facc0[k] = facc1[k] = 0.0;
+    wacc0[k] = wacc1[k] = WideAcc{0, 0, 0};
+  }
+  const int64_t digit_stride = a.lqp * n;
+  const int64_t* tile = a.inter + j * n + static_cast<int64_t>(r) * kTile;
+  const uint32_t pidx = static_cast<uint32_t>(j * n + packed_index(r, 8 * t));
+  // software pipeline: the next digit's key words and values load while
+  // this digit's products run
+  auto load_digit = [&](int64_t di, int64_t (&w0)[8], int64_t (&w1)[8], V (&v)[8]) {
+    Global<int64_t>* k0 = key + key_digit[di] * a.key_sd + grow * a.key_sr + dst_off;
+    Global<int64_t>* k1 = k0 + a.key_sc;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      w0[k] = k0[dst.slot(k)];
+      w1[k] = k1[dst.slot(k)];
+    }
+    if constexpr (kPacked) {
+      const PackedInter pk = packed_slab(a.inter + di * digit_stride, a.small_rows, n);
+      const uint4 l0 = *reinterpret_cast<const uint4*>(pk.lo + pidx);
+      const uint4 l1 = *reinterpret_cast<const uint4*>(pk.lo + pidx + 4);
+      const uint4 hh = *reinterpret_cast<const uint4*>(pk.hi + pidx);
+      const uint32_t lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+      const uint32_t hi2[4] = {hh.x, hh.y, hh.z, hh.w};
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint64_t h = (hi2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+        v[k] = u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32));
+      }
+    } else {
+      const int64_t* tl = tile + di * digit_stride;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = p.unmid(tl[mac_e3(t, k)]);
+    }
+  };
+  int64_t wa0[8], wa1[8], wb0[8], wb1[8];
+  V va[8], vb[8];
+  load_digit(0, wa0, wa1, va);
+  auto mac_digit = [&](int64_t (&w0)[8], int64_t (&w1)[8], V (&v)[8]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if constexpr (kF64) {
+        mac_f64(facc0[k], v[k], w0[k], row.qd, row.qinv);
+        mac_f64(facc1[k], v[k], w1[k], row.qd, row.qinv);
+      } else {
+        wide_mac(wacc0[k], v[k], static_cast<uint64_t>(w0[k]));
+        wide_mac(wacc1[k], v[k], static_cast<uint64_t>(w1[k]));
+      }
+    }
+  };
+#pragma nounroll
+  for (int64_t di = 0; di < a.dnum; di += 2) {
+    if (di + 1 < a.dnum) load_digit(di + 1, wb0, wb1, vb);
+    mac_digit(wa0, wa1, va);
+    if (di + 1 >= a.dnum) break;
+    if (di + 2 < a.dnum) load_digit(di + 2, wa0, wa1, va);
+    mac_digit(wb0, wb1, vb);
+  }
+  if constexpr (kF64) {
+    mac_finish<true>(a, j, b, t, dst, sh, tww, twp, facc0, facc1);
+  } else {
+    // at most 16 products y w with y < 4q, w < 2q: REDC value below 9q
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      uacc0[k] = wide_redc62(wacc0[k], row.q, row.neg_qinv64);
+      uacc1[k] = wide_redc62(wacc1[k], row.q, row.neg_qinv64);
+    }
+    mac_finish<true>(a, j, b, t, dst, sh, tww, twp, uacc0, uacc1);
+  }
+}
+
+template <int MODE, bool kSmall>
+__global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 3 : 2))) void k_ks_mac_hoisted(
+    KsArgs a, int64_t groups, int64_t j_begin, int64_t j_count) {
+  __shared__ int64_t lds[mac_lds_words<kSmall>()];
+  const uint32_t batch = static_cast<uint32_t>(a.batch);
+  const uint32_t jc = static_cast<uint32_t>(j_count);
+  const uint32_t w = xcd_remap(blockIdx.x, static_cast<uint32_t>(groups) * jc * batch);
+  const int64_t b = w % batch;
+  const int64_t j = j_begin + (w / batch) % jc;
+  const int64_t g = w / (batch * jc);
+  const NttRow& row = a.rows[a.level + j];
+  if constexpr (kSmall) {
+    if constexpr (MODE == 1) {
+      tiles_mac_hoisted(F64Fast(row), a, g, j, b, lds);
+    } else {
+      tiles_mac_hoisted(F64Exact(row), a, g, j, b, lds);
+    }
+  } else {
+    tiles_mac_hoisted(U64Fast(row), a, g, j, b, lds);
   }
 }
 
@@ -1584,18 +1853,28 @@ int keyswitch_fused_impl(fh_view3 component, int64_t batch, int64_t lq, int64_t 
     const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
     if (j_count == 0) return kOk;
     const dim3 grid(static_cast<uint32_t>(groups * j_count * batch));
-#define FH_MAC_LAUNCH(H)                                                                                       \
-    if (pass == 1) {                                                                                          \
-      hipLaunchKernelGGL((k_ks_tiles_mac<0, false, H>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count); \
-    } else if (mode == 1) {                                                                                   \
-      hipLaunchKernelGGL((k_ks_tiles_mac<1, true, H>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);  \
-    } else {                                                                                                  \
-      hipLaunchKernelGGL((k_ks_tiles_mac<0, true, H>), grid, dim3(kMacBlock), 0, st, a, groups, j_begin, j_count);  \
+#define FH_MAC_LAUNCH(K, ARGS)                                                                               \
+    if (pass == 1) {                                                                                        \
+      hipLaunchKernelGGL((K<0, false>), grid, dim3(kMacBlock), 0, st, ARGS, groups, j_begin, j_count);      \
+    } else if (mode == 1) {                                                                                 \
+      hipLaunchKernelGGL((K<1, true>), grid, dim3(kMacBlock), 0, st, ARGS, groups, j_begin, j_count);       \
+    } else {                                                                                                \
+      hipLaunchKernelGGL((K<0, true>), grid, dim3(kMacBlock), 0, st, ARGS, groups, j_begin, j_count);       \
     }
     if (hoist) {
-      FH_MAC_LAUNCH(true)
+      // the one input entry's digits: forward tile pass once, in place
+      const dim3 fgrid(static_cast<uint32_t>(groups * j_count * digit_count));
+      if (pass == 1) {
+        hipLaunchKernelGGL((k_ks_tiles_fwd<0, false>), fgrid, dim3(kMacBlock), 0, st, a_in, groups, j_begin, j_count);
+      } else if (mode == 1) {
+        hipLaunchKernelGGL((k_ks_tiles_fwd<1, true>), fgrid, dim3(kMacBlock), 0, st, a_in, groups, j_begin, j_count);
+      } else {
+        hipLaunchKernelGGL((k_ks_tiles_fwd<0, true>), fgrid, dim3(kMacBlock), 0, st, a_in, groups, j_begin, j_count);
+      }
+      FH_CHECK_LAUNCH("hoisted key switch forward tile pass");
+      FH_MAC_LAUNCH(k_ks_mac_hoisted, a)
     } else {
-      FH_MAC_LAUNCH(false)
+      FH_MAC_LAUNCH(k_ks_tiles_mac, a)
     }
 #undef FH_MAC_LAUNCH
     FH_CHECK_LAUNCH("key switch tile pass + MAC");

@@ -42,10 +42,40 @@ typedef struct fh_vec {
   int64_t stride;
 } fh_vec;
 
-/* ---- runtime (replaces csrc/runtime/cuda_info.{h,cpp} inspection) ---- */
+/* ---- runtime (replaces csrc/runtime/cuda_info.{h,cpp} inspection and
+ *      provides the device services of a csrc/runtime backend) ---- */
 const char* fh_last_error(void);
 int fh_abi_version(void);
 int fh_device_info(char* buf, int64_t buf_len);
+/* HIP runtime version of the process (the manifest's hip_version; the
+ * reference records cuda_version, fhelium/native/_abi.py:48-63). */
+int fh_runtime_version(int* hip_runtime_version);
+/* Device-to-device copy with 16-byte vector loads and stores (pointers
+ * 16-byte aligned, non-overlapping); the byte reference of measured HBM
+ * ceilings. */
+int fh_stream_copy(void* dst, const void* src, int64_t bytes, void* stream);
+/* Arena: one device allocation (on the current device) carved into aligned
+ * workspaces by a bump pointer and reset as a whole, so a serving loop or a
+ * captured graph reuses fixed addresses without allocator calls.  It is the
+ * only memory the library allocates, and only in fh_arena_create. */
+typedef struct fh_arena fh_arena;
+int fh_arena_create(int64_t bytes, fh_arena** out);
+int fh_arena_alloc(fh_arena* arena, int64_t bytes, int64_t alignment, void** out);
+int fh_arena_reset(fh_arena* arena);
+int fh_arena_stats(const fh_arena* arena, int64_t* capacity, int64_t* used, int64_t* high_water);
+int fh_arena_destroy(fh_arena* arena);
+/* Timing events (hipEvent_t behind void*). */
+int fh_event_create(void** out);
+int fh_event_record(void* event, void* stream);
+int fh_event_elapsed_ms(void* start, void* stop, float* ms);
+int fh_event_destroy(void* event);
+/* Stream capture of a launch sequence into one replayable graph (the role of
+ * fhelium/execution/cuda_graph.py:58-560 for hosts without PyTorch).
+ * `stream` must not be the null stream. */
+int fh_graph_begin(void* stream);
+int fh_graph_end(void* stream, void** graph_exec);
+int fh_graph_launch(void* graph_exec, void* stream);
+int fh_graph_destroy(void* graph_exec);
 
 /* ---- NTT (csrc/ops/ntt/forward_ntt.cpp:25-80, inverse_ntt.cpp:23-82) ----
  * Backend tables: ntt_rows is one 128-byte record per QP row and

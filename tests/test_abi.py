@@ -77,3 +77,22 @@ def test_host_tensors_have_no_fallback():
     with pytest.raises(NotImplementedError):
         torch.ops.fhelium_ntt_ops.inverse_ntt_to_standard_compact_grouped_smem_(
             x, torch.zeros((2, 8), dtype=torch.int64), params, 0)
+
+
+def test_build_manifest_records_hip_and_detects_mismatch(library):
+    from fhelium_amd import manifest as M
+    path = M.manifest_path(nat.LIB_PATH.parent)
+    body = M.read_manifest(path)
+    assert body["native_backends"] == ["cuda"] and body["offload_arch"] == "gfx950"
+    assert body["hip_version"] and body["torch"]["version"] == torch.__version__
+    assert body["library_abi"] == nat.ABI_VERSION
+    assert M.compatibility_mismatches(body, lib_path=nat.LIB_PATH, library_abi=nat.ABI_VERSION) == []
+    stale = dict(body, library_sha256="0" * 64)
+    assert any("SHA-256" in p for p in M.compatibility_mismatches(stale, lib_path=nat.LIB_PATH,
+                                                                   library_abi=nat.ABI_VERSION))
+    other_torch = dict(body, torch=dict(body["torch"], version="0.0.0"))
+    assert any("torch version" in p for p in M.compatibility_mismatches(other_torch, lib_path=nat.LIB_PATH,
+                                                                         library_abi=nat.ABI_VERSION))
+    bad = dict(body, native_backends=["cuda", "tpu"])
+    assert any("native_backends" in p for p in M.compatibility_mismatches(bad, lib_path=nat.LIB_PATH,
+                                                                           library_abi=nat.ABI_VERSION))
