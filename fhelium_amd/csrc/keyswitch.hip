@@ -117,19 +117,6 @@ struct KsArgs {
   const int64_t* hoist_galois;
 };
 
-// Block i runs on XCD i % 8; give each XCD one contiguous range of work
-// items (batch fastest) so consecutive items on an XCD share key tiles in
-// its L2.  The map is a bijection of [0, total).
-// Block-index arithmetic is 32-bit (grids are below 2^31 blocks): 64-bit
-// division expands to a long instruction sequence on the GPU.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t total) {
-  constexpr uint32_t kXcds = 8;
-  const uint32_t x = i % kXcds;
-  const uint32_t s = i / kXcds;
-  const uint32_t q = total / kXcds;
-  const uint32_t rem = total % kXcds;
-  return x * q + (x < rem ? x : rem) + s;
-}
 
 // Table addresses read from device memory are generic pointers to the
 // compiler, which would then emit FLAT loads; those also count against the

@@ -105,12 +105,12 @@ __device__ __forceinline__ void fwd_columns(const P& p, int64_t* base, const int
 // Forward tile pass: the last 8 stages inside 256-element tiles.  Reads
 // intermediates, writes output residues.
 template <class P>
-__device__ __forceinline__ void fwd_tiles(const P& p, int log_n, int64_t* base, const int64_t* tw,
+__device__ __forceinline__ void fwd_tiles(const P& p, int log_n, int g, int64_t* base, const int64_t* tw,
                                           const int64_t* twp, int64_t* lds) {
   using V = typename P::V;
   const int m = threadIdx.x / 16;
   const int t = threadIdx.x % 16;
-  const int r = blockIdx.x * kTilesPerBlock + m;  // tile index in the row
+  const int r = g * kTilesPerBlock + m;  // tile index in the row
   int64_t* tile = base + static_cast<int64_t>(r) * kTile;
   int64_t* sh = lds + m * kTilePitch;
   const int s0 = log_n - kTileLog;
@@ -155,13 +155,13 @@ static_assert(64 % (kBlock / kTilesPerBlock) == 0, "a tile's threads must lie in
 // Inverse tile pass: stages 0..7 inside 256-element tiles.  Reads input
 // residues, writes intermediates.
 template <class P>
-__device__ __forceinline__ void inv_tiles(const P& p, int log_n, int64_t* base, const int64_t* src,
+__device__ __forceinline__ void inv_tiles(const P& p, int log_n, int g, int64_t* base, const int64_t* src,
                                           const int64_t* tw, const int64_t* twp, bool in_mid,
                                           int64_t* lds) {
   using V = typename P::V;
   const int m = threadIdx.x / 16;
   const int t = threadIdx.x % 16;
-  const int r = blockIdx.x * kTilesPerBlock + m;
+  const int r = g * kTilesPerBlock + m;
   int64_t* tile = base + static_cast<int64_t>(r) * kTile;
   const int64_t* in_tile = src + static_cast<int64_t>(r) * kTile;
   int64_t* sh = lds + m * kTilePitch;
@@ -273,6 +273,12 @@ struct NttArgs {
   int64_t src_batch_stride;
   int64_t src_limb_stride;
   int64_t packed_rows;  // inverse column pass: leading packed binary64 limbs
+  // > 0: the tile passes run on a 1-D grid over (entry, tile group, limb),
+  // remapped so each XCD walks every entry of a (limb, tile group) in turn
+  // and reads that tile group's twiddles from its L2; tile_batch is the
+  // entry count (the launch covers all entries' rows).  0: 2-D grid of
+  // (tile group, flat row from row_offset).
+  int64_t tile_batch;
 };
 
 struct RowRef {
@@ -285,10 +291,9 @@ struct RowRef {
   int64_t limb;
 };
 
-__device__ __forceinline__ RowRef row_ref(const NttArgs& a) {
+__device__ __forceinline__ RowRef row_ref(const NttArgs& a, uint32_t flat) {
   // 32-bit division: flat rows stay below 2^31 (64-bit division is a long
   // instruction sequence on the GPU)
-  const uint32_t flat = static_cast<uint32_t>(a.row_offset) + blockIdx.y;
   const int64_t b = flat / static_cast<uint32_t>(a.limbs);
   const int64_t l = flat - static_cast<uint32_t>(b) * static_cast<uint32_t>(a.limbs);
   const int64_t prow = a.row_start + l;
@@ -296,6 +301,30 @@ __device__ __forceinline__ RowRef row_ref(const NttArgs& a) {
   int64_t* base = a.data + b * a.batch_stride + l * a.limb_stride;
   const int64_t* src = a.src ? a.src + b * a.src_batch_stride + l * a.src_limb_stride : base;
   return RowRef{base, a.rows + prow, a.tw + prow * n, a.twp + prow * n, src, a.data + b * a.batch_stride, l};
+}
+
+// Row of a column-pass block (2-D grid: column group, flat row).
+__device__ __forceinline__ RowRef row_ref(const NttArgs& a) {
+  return row_ref(a, static_cast<uint32_t>(a.row_offset) + blockIdx.y);
+}
+
+// Flat row and tile group of a tile-pass block.
+struct TileItem {
+  uint32_t flat;
+  int g;
+};
+__device__ __forceinline__ TileItem tile_item(const NttArgs& a) {
+  if (a.tile_batch > 0) {
+    const uint32_t groups = static_cast<uint32_t>((int64_t{1} << a.log_n) / (kTile * kTilesPerBlock));
+    const uint32_t entries = static_cast<uint32_t>(a.tile_batch);
+    const uint32_t limbs = static_cast<uint32_t>(a.limbs);
+    const uint32_t w = xcd_remap(blockIdx.x, groups * entries * limbs);
+    const uint32_t b = w % entries;
+    const uint32_t g = (w / entries) % groups;
+    const uint32_t l = w / (entries * groups);
+    return TileItem{b * limbs + l, static_cast<int>(g)};
+  }
+  return TileItem{static_cast<uint32_t>(a.row_offset) + blockIdx.y, static_cast<int>(blockIdx.x)};
 }
 
 enum Mode : int { kExact = 0, kFast = 1 };
@@ -331,16 +360,18 @@ __global__ __launch_bounds__(kBlock) void k_fwd_columns(NttArgs a) {
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_fwd_tiles(NttArgs a) {
   __shared__ int64_t lds[kTilesPerBlock * kTilePitch];
-  const RowRef rr = row_ref(a);
-  FH_DISPATCH_PATH(MODE, *rr.row, (fwd_tiles(p, a.log_n, rr.base, rr.tw, rr.twp, lds)));
+  const TileItem it = tile_item(a);
+  const RowRef rr = row_ref(a, it.flat);
+  FH_DISPATCH_PATH(MODE, *rr.row, (fwd_tiles(p, a.log_n, it.g, rr.base, rr.tw, rr.twp, lds)));
 }
 
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_inv_tiles(NttArgs a) {
   __shared__ int64_t lds[kTilesPerBlock * kTilePitch];
-  const RowRef rr = row_ref(a);
+  const TileItem it = tile_item(a);
+  const RowRef rr = row_ref(a, it.flat);
   FH_DISPATCH_PATH(MODE, *rr.row,
-                   (inv_tiles(p, a.log_n, rr.base, rr.src, rr.tw, rr.twp, a.in_mid != 0, lds)));
+                   (inv_tiles(p, a.log_n, it.g, rr.base, rr.src, rr.tw, rr.twp, a.in_mid != 0, lds)));
 }
 
 template <int S1, int MODE>
@@ -474,7 +505,14 @@ int launch_two_pass(NttArgs a, int64_t total_rows, bool forward, hipStream_t str
     const int64_t rows = total_rows - start < chunk ? total_rows - start : chunk;
     a.row_offset = start;
     const dim3 gcol(kTile / C, static_cast<uint32_t>(rows));
-    const dim3 gtile(static_cast<uint32_t>(n / (kTile * kTilesPerBlock)), static_cast<uint32_t>(rows));
+    const int64_t groups = n / (kTile * kTilesPerBlock);
+    // one launch over every entry: XCD-remapped 1-D tile grid (twiddles
+    // reused from L2 across entries); chunked launches keep the 2-D grid
+    const bool remap = start == 0 && rows == total_rows && total_rows % a.limbs == 0 &&
+                       groups * total_rows <= 0x7fffffff;
+    a.tile_batch = remap ? total_rows / a.limbs : 0;
+    const dim3 gtile = remap ? dim3(static_cast<uint32_t>(groups * total_rows))
+                             : dim3(static_cast<uint32_t>(groups), static_cast<uint32_t>(rows));
     if (forward) {
       launch_columns<MODE>(true, s1, gcol, stream, a);
       FH_CHECK_LAUNCH("ntt forward column pass");

@@ -12,6 +12,21 @@ constexpr int kBlock = 256;
 constexpr int kTileLog = 8;
 constexpr int kTile = 1 << kTileLog;      // contiguous tile of the tile pass
 constexpr int kTilesPerBlock = 16;
+
+// Block i runs on XCD i % 8 (a GPU chiplet with its own L2); give each XCD
+// one contiguous range of work items so consecutive items on an XCD share
+// the data they read again (key tiles, twiddles) in its L2.  The map is a
+// bijection of [0, total).  Block-index arithmetic is 32-bit (grids are
+// below 2^31 blocks): 64-bit division expands to a long instruction
+// sequence on the GPU.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t total) {
+  constexpr uint32_t kXcds = 8;
+  const uint32_t x = i % kXcds;
+  const uint32_t s = i / kXcds;
+  const uint32_t q = total / kXcds;
+  const uint32_t rem = total % kXcds;
+  return x * q + (x < rem ? x : rem) + s;
+}
 constexpr int kTilePitch = kTile + kTile / 16;  // one pad word per 16
 
 __device__ __forceinline__ int tile_pad(int x) { return x + (x >> 4); }

@@ -139,3 +139,22 @@ def compatibility_mismatches(manifest: Mapping[str, Any], *, lib_path: pathlib.P
         for field in fields:
             compare(f"{section} {field}", built.get(field), now.get(field))
     return problems
+
+
+def main(argv: list[str]) -> int:
+    """``python -m fhelium_amd.manifest <library>``: write the manifest for a
+    library built outside ``fhelium_amd.build`` (the CMake build)."""
+
+    from fhelium_amd import _native, build
+
+    if len(argv) < 1:
+        print("usage: python -m fhelium_amd.manifest <libfhelium_amd.so>", file=sys.stderr)
+        return 2
+    lib = pathlib.Path(argv[0]).resolve()
+    print(write_manifest(lib, hip_version=hipcc_version(build.hipcc()), offload_arch=build.OFFLOAD_ARCH,
+                         library_abi=_native.ABI_VERSION))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -343,6 +343,53 @@ def compact_twiddles_montgomery(moduli: tuple[int, ...], log_n: int) -> tuple[to
     return fwd, inv
 
 
+def power_of_two_radix_twiddles(moduli: tuple[int, ...], log_n: int,
+                                radix: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """The reference's fixed-radix NTT tables in Montgomery form
+    (fhelium/engine/ntt/plans/twiddles.py:118-228, converted as its runtime
+    does): forward and inverse outer twists [rows, N-1] and forward and
+    inverse radix-root powers [rows, radix].  Digit d of the transform
+    (stage_start = d * b, b = log2(radix); mirrored for the inverse) stores,
+    for each of its 2^stage_start groups, beta^1 .. beta^(radix-1) with beta
+    = root^{br((1 << (stage_start + b - 1)) + group * radix / 2)}; the root
+    powers are root^{i 2N / radix}."""
+
+    if radix not in (4, 8, 16):
+        raise ValueError("radix must be 4, 8 or 16")
+    bits = {4: 2, 8: 3, 16: 4}[radix]
+    if log_n % bits:
+        raise ValueError("the radix width must divide log_n")
+    n = 1 << log_n
+    order = 2 * n
+
+    def exponents(inverse: bool) -> list[int]:
+        out = []
+        for stage_start in range(0, log_n, bits):
+            first = log_n - stage_start - bits if inverse else stage_start
+            leaf = first + bits - 1
+            for group in range(1 << first):
+                beta = int(f"{(1 << leaf) + group * (radix // 2):0{log_n}b}"[::-1], 2)
+                out.extend(beta * lane % order for lane in range(1, radix))
+        assert len(out) == n - 1
+        return out
+
+    fwd_e, inv_e = exponents(False), exponents(True)
+    rows = len(moduli)
+    fo = torch.empty((rows, n - 1), dtype=torch.int64)
+    io = torch.empty((rows, n - 1), dtype=torch.int64)
+    fr = torch.empty((rows, radix), dtype=torch.int64)
+    ir = torch.empty((rows, radix), dtype=torch.int64)
+    for row, q in enumerate(moduli):
+        psi = primitive_2nth_root(q, n)
+        f = _power_series(psi, order, q)
+        i = _power_series(pow(psi, -1, q), order, q)
+        fo[row] = torch.tensor([f[e] * R % q for e in fwd_e], dtype=torch.int64)
+        io[row] = torch.tensor([i[e] * R % q for e in inv_e], dtype=torch.int64)
+        fr[row] = torch.tensor([f[k * order // radix] * R % q for k in range(radix)], dtype=torch.int64)
+        ir[row] = torch.tensor([i[k * order // radix] * R % q for k in range(radix)], dtype=torch.int64)
+    return fo, io, fr, ir
+
+
 # ---------------------------------------------------------------------------
 # Galois tables
 # ---------------------------------------------------------------------------

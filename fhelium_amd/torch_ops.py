@@ -17,9 +17,14 @@ The compact NTT ops take the reference's table format (compact Montgomery
 twiddles ``[limb, N]`` aligned with the operand's limbs, plus ``rns_params``);
 the backend-specific records those become are derived on the device once per
 (table, parameter) pair and cached.  ``grouped_stage_count`` is a launch
-tuning knob of the reference's CUDA kernel and has no effect here.  The
-power-of-two-radix NTT family is not registered (the engine never selects it
-on this backend).
+tuning knob of the reference's CUDA kernel (validated as 2, 3 or 4, as there)
+and has no effect here.  The power-of-two-radix family takes the reference's
+outer-twist and radix-root tables; the 2N-th root each row's tables are
+built from is read from them, checked against the radix-root powers, and
+expanded on the device into the compact form, so every NTT family runs the
+same two-pass transform.  Forward outputs equal the reference's modulo q
+(its radix-4/8/16 kernels pick their own lazy representatives); standard
+inverse outputs are bit-identical.
 """
 
 from __future__ import annotations
@@ -64,6 +69,20 @@ _NTT_SCHEMAS = [
     "Tensor inverse_twiddles, Tensor rns_params, int grouped_stage_count) -> ()",
     "inverse_ntt_to_centered_compact_grouped_smem_(Tensor(a!) montgomery_residues, "
     "Tensor inverse_twiddles, Tensor rns_params, int grouped_stage_count) -> ()",
+    "forward_ntt_montgomery_power_of_two_radix_compact_(Tensor(a!) montgomery_residues, "
+    "Tensor outer_twiddles, Tensor radix_root_powers, Tensor rns_params) -> ()",
+    "forward_ntt_to_montgomery_power_of_two_radix_compact_(Tensor(a!) standard_residues, "
+    "Tensor outer_twiddles, Tensor radix_root_powers, Tensor rns_params) -> ()",
+    "forward_ntt_to_montgomery_power_of_two_radix_compact(Tensor standard_residues, "
+    "Tensor outer_twiddles, Tensor radix_root_powers, Tensor rns_params) -> Tensor",
+    "inverse_ntt_montgomery_power_of_two_radix_compact_(Tensor(a!) montgomery_residues, "
+    "Tensor outer_twiddles, Tensor radix_root_powers, Tensor rns_params) -> ()",
+    "inverse_ntt_to_standard_lazy_power_of_two_radix_compact_(Tensor(a!) montgomery_residues, "
+    "Tensor outer_twiddles, Tensor radix_root_powers, Tensor rns_params) -> ()",
+    "inverse_ntt_to_standard_power_of_two_radix_compact_(Tensor(a!) montgomery_residues, "
+    "Tensor outer_twiddles, Tensor radix_root_powers, Tensor rns_params) -> ()",
+    "inverse_ntt_to_centered_power_of_two_radix_compact_(Tensor(a!) montgomery_residues, "
+    "Tensor outer_twiddles, Tensor radix_root_powers, Tensor rns_params) -> ()",
 ]
 
 _RNS_SCHEMAS = [
@@ -202,15 +221,24 @@ def _prepared_rows(op: str, twiddles: torch.Tensor, rns_params: torch.Tensor, li
 
 def _compact(op: str, residues: torch.Tensor, twiddles: torch.Tensor, rns_params: torch.Tensor,
              grouped_stage_count: int, forward: bool, form: int) -> None:
+    if grouped_stage_count not in (2, 3, 4):
+        raise RuntimeError("grouped_stage_count must be 2, 3, or 4")
+    view = _residue_view(op, residues)
+    prep = _prepared_rows(op, twiddles, rns_params, view.size(1), view.size(2))
+    _run(view, prep, residues, forward, form)
+
+
+def _residue_view(op: str, residues: torch.Tensor) -> torch.Tensor:
     nat.require_device(residues)
     if residues.dtype != torch.int64:
         raise RuntimeError(f"{op}: the gfx950 backend supports int64 residues")
-    if grouped_stage_count < 0:
-        raise RuntimeError(f"{op}: grouped_stage_count must be non-negative")
     view = nat.batch_view(residues, "residues")
     if view.stride(2) != 1:
         raise RuntimeError(f"{op}: residues need a unit coefficient stride")
-    prep = _prepared_rows(op, twiddles, rns_params, view.size(1), view.size(2))
+    return view
+
+
+def _run(view: torch.Tensor, prep: "_PreparedRows", residues: torch.Tensor, forward: bool, form: int) -> None:
     fn = "fh_ntt_forward" if forward else "fh_ntt_inverse"
     nat.call(fn, view.data_ptr(), view.size(0), view.size(1), prep.log_n, view.stride(0), view.stride(1),
              prep.rows.data_ptr(), prep.w.data_ptr(), prep.wp.data_ptr(), 0, form, _ntt_mode,
@@ -250,9 +278,102 @@ def forward_ntt_montgomery_compact_keyswitch_accumulate_(coefficient_digit_qp, f
     """Forward NTT of the digit in place, then the key MAC
     (forward_ntt_compact_keyswitch_fused.inc.cuh:9-172 semantics)."""
 
-    forward_ntt_montgomery_compact_grouped_smem_(coefficient_digit_qp, forward_twiddles, rns_params, 0)
+    forward_ntt_montgomery_compact_grouped_smem_(coefficient_digit_qp, forward_twiddles, rns_params, 3)
     K.keyswitch_accumulate_digit_products_(accumulator0_qp, accumulator1_qp, coefficient_digit_qp,
                                            key_digit_qp, rns_params, key_row_start)
+
+
+# -- power-of-two-radix NTT ----------------------------------------------------
+def _compact_from_root(root_mont: torch.Tensor, rns_params: torch.Tensor, log_n: int) -> torch.Tensor:
+    """[limb, N] compact Montgomery twiddles root^{br(k)} R from each row's
+    Montgomery root (a [limb] vector), by square-and-multiply on the device."""
+
+    limbs = root_mont.size(0)
+    n = 1 << log_n
+    dev = root_mont.device
+    one = K.montgomery_mul(rns_params[5].reshape(limbs, 1).contiguous(),
+                           torch.ones((limbs, 1), dtype=torch.int64, device=dev), rns_params)  # R mod q
+    k = torch.arange(n, device=dev)
+    br = torch.zeros_like(k)
+    for bit in range(log_n):
+        br |= ((k >> bit) & 1) << (log_n - 1 - bit)
+    acc = one.expand(limbs, n).contiguous()
+    power = root_mont.reshape(limbs, 1).contiguous()
+    for bit in range(log_n):
+        take = ((br >> bit) & 1).bool()
+        prod = K.montgomery_mul(acc, power.expand(limbs, n).contiguous(), rns_params)
+        acc = torch.where(take, prod, acc)
+        power = K.montgomery_mul(power, power, rns_params)
+    return acc
+
+
+_radix_prepared: dict[tuple, tuple] = {}
+
+
+def _radix_rows(op: str, residues: torch.Tensor, outer: torch.Tensor, roots: torch.Tensor,
+                rns_params: torch.Tensor, forward: bool) -> tuple[torch.Tensor, "_PreparedRows"]:
+    """Validate as the reference does (csrc/ops/ntt/ntt_validation.h:123-157)
+    and return the operand view and the prepared rows of these tables."""
+
+    view = _residue_view(op, residues)
+    nat.require_device(outer, roots, rns_params)
+    limbs, n = view.size(1), view.size(2)
+    if outer.dtype != torch.int64 or roots.dtype != torch.int64 or rns_params.dtype != torch.int64:
+        raise RuntimeError(f"{op}: the gfx950 backend supports int64 tables")
+    if outer.dim() != 2 or outer.size(0) != limbs or outer.size(1) != n - 1:
+        raise RuntimeError("Power-of-two radix outer twiddles must have shape [prime, coefficient - 1]")
+    radix = roots.size(1) if roots.dim() == 2 else -1
+    if radix not in (4, 8, 16):
+        raise RuntimeError("Power-of-two radix root width must select radix 4, 8, or 16")
+    if roots.device != residues.device:
+        raise RuntimeError("Power-of-two radix root powers must share the operand device")
+    if roots.size(0) != limbs:
+        raise RuntimeError("Power-of-two radix root powers must have shape [prime, radix]")
+    if K._overlaps(residues, roots):
+        raise RuntimeError("power-of-two radix root powers overlap the residues")
+    log_n = n.bit_length() - 1
+    bits = {4: 2, 8: 3, 16: 4}[radix]
+    if n & (n - 1) or log_n % bits:
+        raise RuntimeError(f"Strict fixed-radix execution requires logN divisible by {bits}; got logN={log_n}")
+    if rns_params.dim() != 2 or rns_params.size(0) < 8 or rns_params.size(1) != limbs:
+        raise RuntimeError(f"{op}: rns_params must be [8, limb] matching the operand")
+    key = (_tensor_key(outer), _tensor_key(roots), _tensor_key(rns_params), forward)
+    with _lock:
+        hit = _radix_prepared.get(key)
+    if hit is None:
+        # the root is the first twist of the digit whose groups start at
+        # stage logN - 1: position N / radix - 1 of a forward table, 0 of an
+        # inverse table
+        root = outer[:, n // radix - 1 if forward else 0].contiguous()
+        compact = _compact_from_root(root, rns_params, log_n)
+        # the table's radix-root powers must be root^(2N/radix) = compact[br(2N/radix)]
+        probe = int(f"{2 * n // radix:0{log_n}b}"[::-1], 2)
+        if not torch.equal(compact[:, probe], roots[:, 1]):
+            raise RuntimeError(f"{op}: outer twiddles and radix root powers describe different roots")
+        prep = _PreparedRows(compact, rns_params)
+        hit = (compact, prep)
+        with _lock:
+            if len(_radix_prepared) > 64:
+                _radix_prepared.clear()
+            _radix_prepared[key] = hit
+    return view, hit[1]
+
+
+def _radix(name: str, forward: bool, form: int):
+    def impl(residues, outer_twiddles, radix_root_powers, rns_params):
+        view, prep = _radix_rows(name, residues, outer_twiddles, radix_root_powers, rns_params, forward)
+        _run(view, prep, residues, forward, form)
+
+    impl.__name__ = name
+    return impl
+
+
+def forward_ntt_to_montgomery_power_of_two_radix_compact(standard_residues, outer_twiddles, radix_root_powers,
+                                                         rns_params):
+    out = standard_residues.clone(memory_format=torch.contiguous_format)
+    _radix("forward_ntt_to_montgomery_power_of_two_radix_compact", True, 1)(out, outer_twiddles,
+                                                                             radix_root_powers, rns_params)
+    return out
 
 
 # -- indexed NTT ---------------------------------------------------------------
@@ -332,9 +453,35 @@ def _implementations() -> dict[str, dict[str, object]]:
         ntt[f"inverse_ntt_{suffix}_indexed_"] = _inverse_indexed(form, f"inverse_ntt_{suffix}_indexed_")
         name = f"inverse_ntt_{suffix}_compact_grouped_smem_"
         ntt[name] = _inverse_compact(form, name)
+        name = f"inverse_ntt_{suffix}_power_of_two_radix_compact_"
+        ntt[name] = _radix(name, False, form)
+    ntt["forward_ntt_montgomery_power_of_two_radix_compact_"] = _radix(
+        "forward_ntt_montgomery_power_of_two_radix_compact_", True, 0)
+    ntt["forward_ntt_to_montgomery_power_of_two_radix_compact_"] = _radix(
+        "forward_ntt_to_montgomery_power_of_two_radix_compact_", True, 1)
+    ntt["forward_ntt_to_montgomery_power_of_two_radix_compact"] = \
+        forward_ntt_to_montgomery_power_of_two_radix_compact
     rns = {name.split("(")[0]: getattr(K, name.split("(")[0]) for name in _RNS_SCHEMAS}
     ckks = {name.split("(")[0]: getattr(K, name.split("(")[0]) for name in _CKKS_SCHEMAS}
     return {"fhelium_ntt_ops": ntt, "fhelium_rns_ops": rns, "fhelium_ckks_ops": ckks}
+
+
+def _fake_impl(name: str, schema: str):
+    """Shape-only implementation for FakeTensor tracing
+    (fhelium/native/wrapper/fake_op.py): in-place ops return nothing,
+    functional ops a fresh tensor of the output shape."""
+
+    if schema.endswith("-> ()"):
+        return lambda *args, **kwargs: None
+    if name == "mixed_radix_basis_extend_to_montgomery":
+        def extend(components, coefficients, rns_params, destination_row_count):
+            return components.new_empty((*components.shape[:-2], destination_row_count, components.shape[-1]))
+        return extend
+    if name == "lift_centered_coefficients":
+        def lift(centered, twice_modulus):
+            return centered.new_empty((*centered.shape[:-1], twice_modulus.shape[0], centered.shape[-1]))
+        return lift
+    return lambda first, *args, **kwargs: torch.empty_like(first, memory_format=torch.contiguous_format)
 
 
 def _existing_schema(ns: str, name: str) -> str | None:
@@ -364,6 +511,8 @@ def register() -> None:
                 existing = _existing_schema(ns, name)
                 if existing is None:
                     defs.define(schema)
+                    # the defining library also describes output shapes
+                    torch.library.register_fake(f"{ns}::{name}", _fake_impl(name, schema), lib=defs)
                 elif existing != f"{ns}::{schema}":
                     raise RuntimeError(f"{ns}::{name} is already defined with a different schema: {existing}")
                 kernels.impl(name, impls[ns][name], "CUDA")

@@ -56,7 +56,7 @@ def test_abi_version(library):
 def test_reference_schemas_registered():
     from fhelium_amd import torch_ops
     ops = torch_ops.registered_operators()
-    assert len(ops) == len(set(ops)) == 51
+    assert len(ops) == len(set(ops)) == 58
     s = torch.ops.fhelium_ntt_ops.forward_ntt_montgomery_compact_keyswitch_accumulate_.default._schema
     assert str(s) == (
         "fhelium_ntt_ops::forward_ntt_montgomery_compact_keyswitch_accumulate_(Tensor(a!) "

@@ -90,7 +90,7 @@ def test_compact_forward_functional_does_not_alias():
     cfg, params, fwd, _ = _tables(12, 0, 39)
     x = H.to_dev(H.random_residues(np.random.default_rng(3), 1, H.moduli_array(cfg), cfg.N))
     before = x.clone()
-    out = NTT.forward_ntt_to_montgomery_compact_grouped_smem(x, fwd.cuda(), params.cuda(), 0)
+    out = NTT.forward_ntt_to_montgomery_compact_grouped_smem(x, fwd.cuda(), params.cuda(), 3)
     assert out.data_ptr() != x.data_ptr()
     assert torch.equal(x, before)
     want = O.ntt_forward(H.to_np(before), fwd.numpy(), params.numpy(), to_montgomery=True)
@@ -170,3 +170,66 @@ def test_rns_and_ckks_ops_through_dispatcher():
     got = CKKS.apply_coefficient_galois_automorphism(H.to_dev(c), idx.cuda(), sgn.cuda(), pd[0].contiguous())
     want = O.galois_coefficient(c, idx.numpy(), sgn.numpy(), params.numpy()[0])
     np.testing.assert_array_equal(H.to_np(got), want)
+
+
+@pytest.mark.parametrize("log_n,radix,row_start", [(12, 4, 0), (12, 8, 3), (12, 16, 0), (16, 16, 0), (16, 4, 30)])
+def test_power_of_two_radix_family(log_n, radix, row_start):
+    """The reference's radix-4/8/16 NTT schemas (forward_ntt.cpp:56-66,
+    inverse_ntt.cpp:58-72) with its outer-twist and radix-root tables:
+    forward outputs equal the oracle's NTT modulo q (as the reference's own
+    cross-family tests compare them, test_ntt_backend.py:350-455), standard
+    and centered inverse outputs bit for bit, lazy inverse outputs modulo q,
+    and the functional form leaves its input alone."""
+    rows = 39 - row_start
+    cfg, params, fwd, inv = _tables(log_n, row_start, rows)
+    fo, io, fr, ir = rns.power_of_two_radix_twiddles(cfg.moduli[row_start: row_start + rows], log_n, radix)
+    q = H.moduli_array(cfg, row_start, rows)[None, :, None]
+    rng = np.random.default_rng(log_n * 100 + radix)
+    x = H.random_residues(rng, 2, q.ravel(), cfg.N, lazy=True)
+    p, fo_d, fr_d, io_d, ir_d = params.cuda(), fo.cuda(), fr.cuda(), io.cuda(), ir.cuda()
+    print(f"radix family logN={log_n} radix={radix} rows={rows}", flush=True)
+    dev = H.to_dev(x)
+    NTT.forward_ntt_montgomery_power_of_two_radix_compact_(dev, fo_d, fr_d, p)
+    want = O.ntt_forward(x, fwd.numpy(), params.numpy(), to_montgomery=False)
+    np.testing.assert_array_equal(H.to_np(dev) % q, want % q)
+    xs = x % q
+    dev = H.to_dev(xs)
+    NTT.forward_ntt_to_montgomery_power_of_two_radix_compact_(dev, fo_d, fr_d, p)
+    want = O.ntt_forward(xs, fwd.numpy(), params.numpy(), to_montgomery=True)
+    np.testing.assert_array_equal(H.to_np(dev) % q, want % q)
+    src = H.to_dev(xs)
+    out = NTT.forward_ntt_to_montgomery_power_of_two_radix_compact(src, fo_d, fr_d, p)
+    assert torch.equal(src, H.to_dev(xs)) and out.data_ptr() != src.data_ptr()
+    np.testing.assert_array_equal(H.to_np(out) % q, want % q)
+    for form, suffix in enumerate(["montgomery", "to_standard_lazy", "to_standard", "to_centered"]):
+        dev = H.to_dev(x)
+        getattr(NTT, f"inverse_ntt_{suffix}_power_of_two_radix_compact_")(dev, io_d, ir_d, p)
+        want = O.ntt_inverse(x, inv.numpy(), params.numpy(), form)
+        if form >= 2:
+            np.testing.assert_array_equal(H.to_np(dev), want, err_msg=suffix)
+        else:
+            np.testing.assert_array_equal(H.to_np(dev) % q, want % q, err_msg=suffix)
+    # round trip: standard -> NTT -> standard is the identity on canonical residues
+    dev = H.to_dev(xs)
+    NTT.forward_ntt_to_montgomery_power_of_two_radix_compact_(dev, fo_d, fr_d, p)
+    NTT.inverse_ntt_to_standard_power_of_two_radix_compact_(dev, io_d, ir_d, p)
+    np.testing.assert_array_equal(H.to_np(dev), xs)
+
+
+def test_power_of_two_radix_validation():
+    cfg, params, _, _ = _tables(16, 0, 39)
+    fo, io, fr, ir = rns.power_of_two_radix_twiddles(cfg.moduli, 16, 16)
+    x = torch.zeros((1, 39, cfg.N), dtype=torch.int64, device="cuda")
+    p = params.cuda()
+    with pytest.raises(RuntimeError, match="radix 4, 8, or 16"):
+        NTT.forward_ntt_montgomery_power_of_two_radix_compact_(x, fo.cuda(), fr[:, :5].contiguous().cuda(), p)
+    with pytest.raises(RuntimeError, match="coefficient - 1"):
+        NTT.forward_ntt_montgomery_power_of_two_radix_compact_(x, fo[:, :-1].contiguous().cuda(), fr.cuda(), p)
+    with pytest.raises(RuntimeError, match="different roots"):
+        NTT.forward_ntt_montgomery_power_of_two_radix_compact_(x, fo.cuda(), ir.cuda(), p)
+    _, _, fr8, _ = rns.power_of_two_radix_twiddles(cfg.moduli[:2], 12, 8)
+    with pytest.raises(RuntimeError, match="divisible by 3"):
+        NTT.forward_ntt_montgomery_power_of_two_radix_compact_(
+            x, fo.cuda(), torch.zeros((39, 8), dtype=torch.int64, device="cuda"), p)
+    with pytest.raises(RuntimeError, match="grouped_stage_count"):
+        NTT.forward_ntt_montgomery_compact_grouped_smem_(x, fo.cuda(), p, 5)
