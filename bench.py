@@ -69,6 +69,9 @@ def parse_args():
     p.add_argument("--workload", default="batched", choices=["batched", "ntt", "matvec", "serving"],
                    help="batched: the headline line (uneven HMult + rotation batch, with every side "
                         "measurement); ntt / matvec / serving: only that configuration's measurement")
+    p.add_argument("--graph", dest="graph", action="store_true", default=True,
+                   help="also time the batched step as one captured HIP graph (default on)")
+    p.add_argument("--no-graph", dest="graph", action="store_false")
     p.add_argument("--calibration-copy", action="store_true",
                    help="run one 256 MiB device copy (torch.neg) before timing, as the byte "
                         "reference for hardware-counter traffic figures")
@@ -137,6 +140,35 @@ def run_batched(eng, groups, lat_h, lat_r):
             lat_r.append((e1, e2, len(g["steps"])))
         outs.append(r)
     return outs
+
+
+def graph_leg(eng, groups, steps: int, eager_outs) -> dict:
+    """The whole batched step (every level group's HMult and rotations)
+    captured as one HIP graph (fhelium_amd.graph.CapturedStep) and replayed
+    on the resident inputs; the first replay must equal the eager outputs
+    bit for bit."""
+
+    from fhelium_amd.graph import CapturedStep
+
+    items = list(groups.items())
+    inputs = [ct for _, g in items for ct in (g["a"], g["b"])]
+
+    def step(*cts):
+        return [eng.rotate_by_steps(eng.hmult(cts[2 * i], cts[2 * i + 1]), g["steps"])
+                for i, (_, g) in enumerate(items)]
+
+    captured = CapturedStep(step, *inputs)
+    captured.graph.replay()
+    torch.cuda.synchronize()
+    exact = all(torch.equal(o.data, e.data) for o, e in zip(captured.static_output, eager_outs))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        captured.graph.replay()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    del captured
+    return {"path": "whole batched step as one captured HIP graph, replayed on resident inputs",
+            "ms_per_step": ms, "bit_exact_vs_eager": exact}
 
 
 def timed(fn, steps: int, warmup: int, before_timing=None):
@@ -267,9 +299,19 @@ def copy_ceiling(reps: int = 5) -> dict:
 def ntt_measure(eng, reps: int = 5) -> dict:
     """Forward (to Montgomery) and inverse (to standard) NTT of the ring-kernel
     shape [64 ciphertexts, 2 polys, 35 limbs, 2^16], uniform residues,
-    out of place; algorithmic bytes = one read and one write of every limb."""
+    out of place; algorithmic bytes = one read and one write of every limb.
+    The launch schedule of this shape is autotuned first (fhelium_amd/
+    autotune.py; the chosen schedules are reported)."""
+
+    from fhelium_amd import autotune
 
     cfg = eng.config
+    be = eng.runtime.ntt
+    if be.tuning is None:
+        be.tuning = autotune.TuningCache()
+    tuned = {d: autotune.tune_ntt(be, 128, cfg.num_q_primes, 0, forward=(d == "forward"))
+             for d in ("forward", "inverse")}
+    log(f"ntt autotune: forward {tuned['forward']['choice']}, inverse {tuned['inverse']['choice']}")
     q = torch.tensor(cfg.moduli[: cfg.num_q_primes], dtype=torch.int64, device="cuda").view(1, 1, -1, 1)
     x = torch.randint(0, 2**62, (64, 2, cfg.num_q_primes, cfg.N), dtype=torch.int64, device="cuda") % q
     ntt = eng.runtime.ntt
@@ -296,6 +338,7 @@ def ntt_measure(eng, reps: int = 5) -> dict:
         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
         "algorithmic_bytes_per_call": nbytes,
         "forward_ms": out["forward"]["ms"], "inverse_ms": out["inverse"]["ms"],
+        "autotune": {d: {"choice": r["choice"], "candidates_ms": r["candidates_ms"]} for d, r in tuned.items()},
     }
 
 
@@ -357,7 +400,7 @@ def serving_run(eng, args) -> dict:
         return [Request(i, pool[int(levels[i])][i % len(pool[int(levels[i])])][1], float(times[i]))
                 for i in range(count)]
 
-    loop = ServingLoop(prog, max_batch=16, max_active=64)
+    loop = ServingLoop(prog, max_batch=32, max_active=64)
     loop.run(requests(min(n, 8), np.zeros(n)))  # warm-up: kernels and constant plaintexts
     torch.cuda.synchronize()
     reqs = requests(n, arrivals)
@@ -474,6 +517,12 @@ def main() -> int:
     ks_rec = eng.key_switcher.recorder
     eng.key_switcher.recorder = None
     log(f"batched {ms_batch:.2f} ms/step")
+    graph = None
+    if args.graph:
+        graph = graph_leg(eng, groups, args.steps, outs)
+        graph["value"] = 2 * args.count / (graph["ms_per_step"] * 1e-3)
+        graph["vs_eager"] = ms_batch / graph["ms_per_step"]
+        log(f"graph replay {graph['ms_per_step']:.2f} ms/step (bit-exact vs eager: {graph['bit_exact_vs_eager']})")
 
     n_ops = 2 * args.count
     plain_h = [a.elapsed_time(b) for a, b in plh]
@@ -528,6 +577,7 @@ def main() -> int:
             "hmult_per_s": args.count / (sum(a.elapsed_time(b) for a, b in ubh) / args.steps * 1e-3),
             "rotations_per_s": args.count / (sum(a.elapsed_time(b) for a, b in ubr) / args.steps * 1e-3),
         },
+        "graph_replay": graph,
         "batched_vs_plain": None if args.skip_plain else ms_plain / ms_batch,
         "batched_vs_unbatched_fused": None if args.skip_plain else ms_unb / ms_batch,
         "p50_ms": percentile(batch_h + batch_r, 50),

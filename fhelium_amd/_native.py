@@ -17,7 +17,7 @@ import threading
 import torch
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "_lib" / "libfhelium_amd.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 
 class NativeExtensionError(RuntimeError):
@@ -57,7 +57,6 @@ SIGNATURES: dict[str, list] = {
                             _i32, _i32, _ptr],
     "fh_ntt_inverse_from": [_ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64,
                             _i32, _i32, _ptr],
-    "fh_ntt_set_chunk_rows": [_i64],
     "fh_ntt_indexed_stage": [fh_view3, _i64, _i64, _ptr, _ptr, _i64, _ptr, _i64, _i64, fh_table2, _i64, _i32, _ptr],
     "fh_ntt_prepare_tables": [_ptr, _i64, _i64, _i64, _i64, _ptr, _i64, _ptr, _ptr, _ptr, _ptr],
     "fh_rns_binary": [_i32, fh_view3, fh_view3, fh_view3, fh_table2, _i64, _i64, _i64, _i64, _ptr],

@@ -468,9 +468,6 @@ __global__ __launch_bounds__(kBlock) void k_scale(NttArgs a, int what) {
 }
 
 // Rows per launch of the two-pass schedule (0: as many as the grid allows).
-// Measured on MI355X, splitting a transform into launches whose
-// intermediates would fit the Infinity Cache is slower than one launch.
-int64_t g_chunk_rows = 0;
 
 template <int MODE>
 void launch_columns(bool forward, int s1, dim3 grid, hipStream_t stream, const NttArgs& a) {
@@ -500,7 +497,9 @@ int launch_two_pass(NttArgs a, int64_t total_rows, bool forward, hipStream_t str
   const int s1 = L - kTileLog;
   const int T = (1 << s1) / 16;
   const int C = kBlock / T;
-  const int64_t chunk = g_chunk_rows > 0 ? g_chunk_rows : (total_rows < 65535 ? total_rows : 65535);
+  // the column pass's 2-D grid holds at most 65535 rows per launch; finer
+  // schedules (chunks on several streams) are the caller's choice per call
+  const int64_t chunk = total_rows < 65535 ? total_rows : 65535;
   for (int64_t start = 0; start < total_rows; start += chunk) {
     const int64_t rows = total_rows - start < chunk ? total_rows - start : chunk;
     a.row_offset = start;
@@ -796,12 +795,6 @@ FH_API int fh_ntt_inverse_from(int64_t* residues, const int64_t* source, int64_t
   return launch(a, batch * limbs, false, mode, static_cast<hipStream_t>(stream));
 }
 
-// Rows per launch of the two-pass schedule (0 = all rows in one launch).
-FH_API int fh_ntt_set_chunk_rows(int64_t rows) {
-  FH_CHECK_ARG(rows >= 0 && rows <= 65535, "chunk rows must be in [0, 65535]");
-  g_chunk_rows = rows;
-  return kOk;
-}
 
 FH_API int fh_ntt_prepare_tables(const int64_t* rns_params, int64_t param_row_stride,
                                  int64_t param_limb_stride, int64_t limbs, int64_t log_n,

@@ -35,7 +35,7 @@ class CkksEngine:
     def __init__(self, preset: Preset | str | CkksConfig = Preset.slots32768_scale40_levels34_int64,
                  *, device: str | torch.device = "cuda", ntt_mode: str = "fast",
                  rng_seed: int | None = None, rng_nonce: int | None = None,
-                 allow_sk_gen: bool = True, fused_keyswitch: bool = True) -> None:
+                 allow_sk_gen: bool = True, fused_keyswitch: bool = True, ntt_tuning=None) -> None:
         self.config = preset if isinstance(preset, CkksConfig) else CkksConfig.from_preset(preset)
         # rng_seed / rng_nonce fix the stream for tests and benchmarks only;
         # left unset, the key and nonce come from the operating system's
@@ -44,7 +44,8 @@ class CkksEngine:
             raise TypeError("rng_seed must be an integer or None")
         if rng_nonce is not None and type(rng_nonce) is not int:
             raise TypeError("rng_nonce must be an integer or None")
-        self.runtime = RnsRuntime(self.config, device, ntt_mode)
+        # ntt_tuning: an autotune.TuningCache of tuned NTT launch schedules
+        self.runtime = RnsRuntime(self.config, device, ntt_mode, ntt_tuning)
         self.device = self.runtime.device
         self.layout = self.runtime.layout
         self.key_switcher = HybridKeySwitcher(self.runtime)

@@ -69,17 +69,23 @@ class Gfx950NttBackend:
 
     name = "gfx950_radix16_lds"
 
-    def __init__(self, tables: NttTables, mode: str = "fast") -> None:
+    def __init__(self, tables: NttTables, mode: str = "fast", *, moduli: tuple[int, ...] | None = None,
+                 tuning=None) -> None:
         if mode not in ("fast", "exact"):
             raise ValueError("mode must be 'fast' or 'exact'")
         self.tables = tables
         self.mode = mode
         self._mode = MODE_FAST if mode == "fast" else MODE_EXACT
+        self.moduli = moduli
+        # Optional autotune.TuningCache: per-shape launch schedules.  Without
+        # one (or for an untuned shape) every transform is one launch.
+        self.tuning = tuning
+        self._side_streams: dict[int, list] = {}
         # Optional list receiving (start_event, end_event, algorithmic_bytes)
         # per transform, recorded on the operand's stream (roofline probes).
         self.recorder: list | None = None
 
-    def _run(self, fn: str, operand: torch.Tensor, row_start: int, mode: int) -> None:
+    def _check(self, operand: torch.Tensor, row_start: int) -> torch.Tensor:
         nat.require_device(operand)
         if operand.dtype != torch.int64:
             raise TypeError("the gfx950 backend supports int64 residues")
@@ -94,17 +100,71 @@ class Gfx950NttBackend:
         view = nat.batch_view(operand, "residues")
         if view.stride(2) != 1:
             raise ValueError("NTT operands need a unit coefficient stride")
-        fwd = fn == "fh_ntt_forward"
+        return view
+
+    def schedule_for(self, entries: int, limbs: int, forward: bool, device: torch.device):
+        """The tuned launch schedule of this shape (one launch if untuned)."""
+
+        from fhelium_amd import autotune
+
+        if self.tuning is None or entries < 2:
+            return autotune.ONE_LAUNCH
+        key = autotune.ntt_key(device, self.tables.log_n, limbs, entries, forward, self._mode)
+        return autotune.schedule_from_record(self.tuning.get(key))
+
+    def _streams(self, device: torch.device, count: int) -> list:
+        idx = device.index if device.index is not None else torch.cuda.current_device()
+        pool = self._side_streams.setdefault(idx, [])
+        while len(pool) < count:
+            pool.append(torch.cuda.Stream(device=idx))
+        return pool[:count]
+
+    def _launch(self, fn: str, dst: torch.Tensor, src: torch.Tensor | None, row_start: int, form: int) -> None:
+        t = self.tables
+        fwd = fn.startswith("fh_ntt_forward")
+        w = (t.fwd if fwd else t.inv).data_ptr()
+        wp = (t.fwd_shoup if fwd else t.inv_shoup).data_ptr()
+        s = nat.stream_ptr(dst.device)
+        if src is None:
+            nat.call(fn, dst.data_ptr(), dst.size(0), dst.size(1), t.log_n, dst.stride(0), dst.stride(1),
+                     t.rows.data_ptr(), w, wp, row_start, form, self._mode, s)
+        else:
+            nat.call(fn, dst.data_ptr(), src.data_ptr(), src.size(0), src.size(1), t.log_n, dst.stride(0),
+                     dst.stride(1), src.stride(0), src.stride(1), t.rows.data_ptr(), w, wp, row_start, form,
+                     self._mode, s)
+
+    def _apply(self, view: torch.Tensor, row_start: int, forward: bool, form: int, schedule=None,
+               src: torch.Tensor | None = None) -> None:
+        """Transform the [B, limb, N] ``view`` (in place, or from ``src``)
+        with the given schedule: one launch, or chunks of entries issued
+        round-robin on side streams forked from and joined to the current
+        stream.  Results do not depend on the schedule."""
+
+        fn = ("fh_ntt_forward" if forward else "fh_ntt_inverse") + ("_from" if src is not None else "")
+        if schedule is None:
+            schedule = self.schedule_for(view.size(0), view.size(1), forward, view.device)
+        chunk = schedule.chunk_entries
+        if chunk is None or chunk >= view.size(0):
+            self._launch(fn, view, src, row_start, form)
+            return
+        cur = torch.cuda.current_stream(view.device)
+        streams = self._streams(view.device, schedule.streams)
+        for st in streams:
+            st.wait_stream(cur)
+        for i, b in enumerate(range(0, view.size(0), chunk)):
+            with torch.cuda.stream(streams[i % len(streams)]):
+                self._launch(fn, view[b: b + chunk], None if src is None else src[b: b + chunk], row_start, form)
+        for st in streams:
+            cur.wait_stream(st)
+
+    def _run(self, fn: str, operand: torch.Tensor, row_start: int, mode: int, schedule=None) -> None:
+        view = self._check(operand, row_start)
         rec = self.recorder
         if rec is not None:
             start = torch.cuda.Event(enable_timing=True)
             end = torch.cuda.Event(enable_timing=True)
             start.record()
-        nat.call(fn, view.data_ptr(), view.size(0), view.size(1), t.log_n, view.stride(0),
-                 view.stride(1), t.rows.data_ptr(),
-                 (t.fwd if fwd else t.inv).data_ptr(),
-                 (t.fwd_shoup if fwd else t.inv_shoup).data_ptr(),
-                 row_start, mode, self._mode, nat.stream_ptr(operand.device))
+        self._apply(view, row_start, fn == "fh_ntt_forward", mode, schedule)
         if rec is not None:
             end.record()
             # algorithmic bytes: every residue read once and written once
@@ -136,27 +196,15 @@ class Gfx950NttBackend:
         """Out-of-place transform into a new contiguous tensor; ``operand``
         is left unchanged (the first pass reads it, so no clone is made)."""
 
-        nat.require_device(operand)
-        t = self.tables
-        if operand.dtype != torch.int64 or operand.size(-1) != (1 << t.log_n):
-            raise ValueError("operand must be int64 [..., limb, N] for this ring")
-        if row_start < 0 or row_start + operand.size(-2) > t.rows.size(0):
-            raise ValueError("NTT operand rows exceed the prepared canonical parameter range")
-        src = nat.batch_view(operand, "residues")
-        if src.stride(2) != 1:
-            raise ValueError("NTT operands need a unit coefficient stride")
+        src = self._check(operand, row_start)
         out = torch.empty(operand.shape, dtype=operand.dtype, device=operand.device)
         dst = out.view(src.shape)
-        fwd = fn == "fh_ntt_forward_from"
         rec = self.recorder
         if rec is not None:
             start = torch.cuda.Event(enable_timing=True)
             end = torch.cuda.Event(enable_timing=True)
             start.record()
-        nat.call(fn, dst.data_ptr(), src.data_ptr(), src.size(0), src.size(1), t.log_n,
-                 dst.stride(0), dst.stride(1), src.stride(0), src.stride(1), t.rows.data_ptr(),
-                 (t.fwd if fwd else t.inv).data_ptr(), (t.fwd_shoup if fwd else t.inv_shoup).data_ptr(),
-                 row_start, form, self._mode, nat.stream_ptr(operand.device))
+        self._apply(dst, row_start, fn == "fh_ntt_forward_from", form, None, src=src)
         if rec is not None:
             end.record()
             rec.append((start, end, 2 * src.size(0) * src.size(1) * src.size(2) * 8))

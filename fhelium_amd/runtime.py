@@ -44,7 +44,7 @@ class DeviceDigitTables:
 
 class RnsRuntime:
     def __init__(self, config: CkksConfig, device: torch.device | str = "cuda",
-                 ntt_mode: str = "fast") -> None:
+                 ntt_mode: str = "fast", ntt_tuning=None) -> None:
         self.config = config
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -59,7 +59,10 @@ class RnsRuntime:
         self.inverse_twiddles = inv.to(dev)
         self.ntt_tables = gntt.NttTables(self.rns_params, self.forward_twiddles,
                                          self.inverse_twiddles, config.logN)
-        self.ntt = gntt.Gfx950NttBackend(self.ntt_tables, ntt_mode)
+        # ntt_tuning: an autotune.TuningCache whose per-shape schedules the
+        # NTT backend follows (None: one launch per transform)
+        self.ntt = gntt.Gfx950NttBackend(self.ntt_tables, ntt_mode, moduli=tuple(config.moduli),
+                                         tuning=ntt_tuning)
         self.digit_tables = {
             key: DeviceDigitTables.from_host(key, t, dev) for key, t in self.host.digit_tables.items()
         }

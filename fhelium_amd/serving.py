@@ -14,8 +14,10 @@ tier built on ``CkksEngine``:
   PyTorch's cached-but-unused blocks, minus a reserve, read when the loop
   starts.
 * **Continuous batching.**  Every iteration groups the active requests by
-  (stage, starting level) -- one group has one tensor shape -- and runs each
-  group's stage as batched engine calls of at most ``max_batch`` entries.  A
+  the program's batch key -- (stage, starting level) for most stages, one
+  tensor shape per group; the rotate-and-add stages of every request at one
+  level form one group whatever their step (per-entry rotation keys) -- and
+  runs each group as batched engine calls of at most ``max_batch`` entries.  A
   request admitted while others are part-way through their programs joins
   the next iteration, so its stages run in the same iteration as theirs:
   batched with them where the stage and level agree, back to back on the
@@ -159,10 +161,12 @@ class PolynomialWindowSum:
 
     # -- batched building blocks --------------------------------------------
     def _hmult(self, lhs: list[Ciphertext], rhs: list[Ciphertext] | None) -> list[Ciphertext]:
-        eng = self.engine
-        a = eng.coefficient_domain_to_ntt_domain(stack(lhs))
-        b = a if rhs is None else eng.coefficient_domain_to_ntt_domain(stack(rhs))
-        return unstack(eng.rescale_to_next_level(eng.relinearize(eng.multiply(a, b))))
+        """Batched HMult + relinearize + rescale: the engine's fused HMult
+        (bit-identical to NTT, multiply, relinearize, rescale in turn)."""
+
+        a = stack(lhs)
+        b = a if rhs is None else stack(rhs)
+        return unstack(self.engine.hmult(a, b))
 
     def _const_mult(self, cts: list[Ciphertext], value: float, target_scale: float) -> Ciphertext:
         """Batched value * ct, rescaled, landing exactly on ``target_scale``."""
@@ -177,6 +181,28 @@ class PolynomialWindowSum:
 
     def _at(self, cts: list[Ciphertext], level: int) -> list[Ciphertext]:
         return [c if c.level == level else self.engine.mod_switch_to_level(c, level) for c in cts]
+
+    # -- batching ---------------------------------------------------------------
+    def batch_key(self, stage: int, start_level: int) -> tuple:
+        """Requests whose current stages share a key run as one batch.  The
+        rotate-and-add stages of every request at one level share a key
+        whatever their step (``rotate_by_steps`` takes a step per entry);
+        the other stages batch by (stage, start level)."""
+
+        if stage >= 5:
+            return ("rotate", start_level + self.depth)
+        return (stage, start_level)
+
+    def run_group(self, key: tuple, group: list["Request"]) -> None:
+        if key[0] != "rotate":
+            self.run_stage(key[0], group)
+            return
+        eng = self.engine
+        steps = [self.rotation_steps[r.stage - 5] for r in group]
+        s = stack([r.registers["s"] for r in group])
+        s = eng.add(s, eng.rotate_by_steps(s, steps), inplace=True)
+        for r, v in zip(group, unstack(s)):
+            r.registers["s"] = v
 
     # -- stages ----------------------------------------------------------------
     def run_stage(self, stage: int, group: list["Request"]) -> None:
@@ -373,15 +399,28 @@ class ServingLoop:
                 continue
             max_active_seen = max(max_active_seen, len(active))
             peak_committed = max(peak_committed, committed)
-            groups: dict[tuple[int, int], list[Request]] = {}
+            # batching groups: the program's own keys where it defines them
+            # (its rotate-and-add stages batch across requests' stages),
+            # otherwise one group per (stage, start level)
+            key_of = getattr(prog, "batch_key", None)
+            run_group = getattr(prog, "run_group", None)
+            groups: dict[tuple, list[Request]] = {}
             for r in active:
-                groups.setdefault((r.stage, r.start_level), []).append(r)
-            for (stage, level), members in sorted(groups.items(), key=lambda kv: (-kv[0][0], kv[0][1])):
+                key = key_of(r.stage, r.start_level) if key_of else (r.stage, r.start_level)
+                groups.setdefault(key, []).append(r)
+
+            def order(key: tuple) -> tuple:
+                return (-key[0] if isinstance(key[0], int) else -prog.stage_count, key[1])
+
+            for key, members in sorted(groups.items(), key=lambda kv: order(kv[0])):
                 for i in range(0, len(members), self.max_batch):
                     chunk = members[i: i + self.max_batch]
                     max_batch_seen = max(max_batch_seen, len(chunk))
-                    self.trace.append((iterations, stage, level, tuple(r.rid for r in chunk)))
-                    prog.run_stage(stage, chunk)
+                    self.trace.append((iterations, key[0], key[1], tuple(r.rid for r in chunk)))
+                    if run_group is not None:
+                        run_group(key, chunk)
+                    else:
+                        prog.run_stage(key[0], chunk)
             still: list[Request] = []
             for r in active:
                 r.stage += 1

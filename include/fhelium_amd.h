@@ -90,9 +90,13 @@ int fh_ntt_prepare_tables(const int64_t* rns_params, int64_t param_row_stride,
                           const int64_t* montgomery_twiddles, int64_t twiddle_row_stride,
                           void* ntt_rows_out, int64_t* twiddles_out,
                           int64_t* twiddles_shoup_out, void* stream);
-/* forward_ntt_{montgomery,to_montgomery}_compact_grouped_smem_ and the
- * indexed / power-of-two families: to_montgomery = 1 consumes standard
- * residues (x -> xR first). */
+/* forward_ntt_{montgomery,to_montgomery}_compact_grouped_smem_ and
+ * forward_ntt_*_power_of_two_radix_compact_ (whose outer-twist tables the
+ * torch shim turns into these compact tables): to_montgomery = 1 consumes
+ * standard residues (x -> xR first).  Every call is one launch per pass
+ * (columns, then tiles) over all batch entries; a caller may cut a batch
+ * into several calls on several streams (fhelium_amd/autotune.py picks that
+ * schedule per shape). */
 int fh_ntt_forward(int64_t* residues, int64_t batch, int64_t limbs, int64_t log_n,
                    int64_t batch_stride, int64_t limb_stride, const void* ntt_rows,
                    const int64_t* twiddles, const int64_t* twiddles_shoup, int64_t row_start,
@@ -130,10 +134,6 @@ int fh_ntt_indexed_stage(fh_view3 residues, int64_t batch, int64_t limbs,
                          int64_t index_stride, const int64_t* stage_twiddles,
                          int64_t twiddle_limb_stride, int64_t twiddle_butterfly_stride,
                          fh_table2 rns_params, int64_t butterflies, int32_t forward, void* stream);
-/* Tuning knob (no reference counterpart; the reference's smem-budget
- * diagnostic ops play this role, forward_ntt.cpp:71-80): rows per launch of
- * the two-pass schedule, 0 = as many as the grid allows. */
-int fh_ntt_set_chunk_rows(int64_t rows);
 
 /* ---- RNS arithmetic (csrc/ops/rns/rns_arithmetic.cpp:30-72,
  *      rns_canonical_arithmetic.cpp:15-23) ----

@@ -70,14 +70,15 @@ def test_inverse_bit_exact(log_n, out_form, mode):
     np.testing.assert_array_equal(H.to_np(dev), want)
 
 
-@pytest.mark.parametrize("chunk_rows", [0, 5, 128])
-def test_row_start_and_strided_batch(chunk_rows):
-    """A level-5 QP interval (rows 5..38) inside a strided batch view, with
-    the two-pass schedule split into chunks that cut across batch entries."""
-    from fhelium_amd import _native as nat
+@pytest.mark.parametrize("schedule", [None, (1, 2), (2, 3)])
+def test_row_start_and_strided_batch(schedule):
+    """A level-5 QP interval (rows 5..38) inside a strided batch view, as one
+    launch or cut into chunks of entries on several streams (the autotuner's
+    schedules): identical results."""
+    from fhelium_amd.autotune import ONE_LAUNCH, NttSchedule
     log_n = 13
     cfg, params, fwd, inv, be = _backend(log_n)
-    nat.call("fh_ntt_set_chunk_rows", chunk_rows)
+    sched = ONE_LAUNCH if schedule is None else NttSchedule(*schedule)
     rng = np.random.default_rng(7)
     start = 5
     mods = H.moduli_array(cfg, start)
@@ -85,16 +86,38 @@ def test_row_start_and_strided_batch(chunk_rows):
     big = torch.zeros((3, 2, len(mods), cfg.N), dtype=torch.int64, device="cuda")
     big[:, 1] = H.to_dev(x)
     view = big[:, 1]
-    be.forward_to_montgomery_(view, start)
+    be._apply(view, start, True, 1, sched)
     sub = slice(start, None)
     want = O.ntt_forward(x, np.ascontiguousarray(fwd[sub]), np.ascontiguousarray(params[:, sub]), True)
     np.testing.assert_array_equal(H.to_np(view), want)
     assert int(big[:, 0].abs().sum()) == 0
-    be.inverse_to_standard_(view, start)
+    be._apply(view, start, False, 2, sched)
     back = O.ntt_inverse(want, np.ascontiguousarray(inv[sub]), np.ascontiguousarray(params[:, sub]), 2)
     np.testing.assert_array_equal(H.to_np(view), back)
     np.testing.assert_array_equal(back, x)  # to-Montgomery then standard inverse is the identity
-    nat.call("fh_ntt_set_chunk_rows", 0)
+
+
+def test_autotuned_schedule_is_persisted_and_exact(tmp_path):
+    """tune_ntt times the candidate schedules of a shape, stores the fastest
+    in the JSON cache, and the backend then uses it; results are the same
+    as one launch."""
+    from fhelium_amd import autotune
+    from fhelium_amd.runtime import RnsRuntime
+    cfg = H.config(14)
+    rt = RnsRuntime(cfg, "cuda", "fast")
+    be = rt.ntt
+    be.tuning = autotune.TuningCache(tmp_path / "tuning.json")
+    rec = autotune.tune_ntt(be, 8, 35, 0, forward=True, reps=3)
+    assert rec["choice"] in rec["candidates_ms"] and len(rec["candidates_ms"]) >= 3
+    again = autotune.TuningCache(tmp_path / "tuning.json")
+    key = autotune.ntt_key(torch.device("cuda", 0), 14, 35, 8, True, be._mode)
+    assert again.get(key)["schedule"] == rec["schedule"]
+    assert be.schedule_for(8, 35, True, torch.device("cuda", 0)) == autotune.schedule_from_record(rec)
+    q = torch.tensor(cfg.moduli[:35], dtype=torch.int64, device="cuda").view(1, -1, 1)
+    x = torch.randint(0, 1 << 62, (8, 35, cfg.N), dtype=torch.int64, device="cuda") % q
+    tuned = be.forward_to_montgomery(x, 0)
+    be.tuning = None
+    assert torch.equal(tuned, be.forward_to_montgomery(x, 0))
 
 
 @pytest.mark.parametrize("mode", ["exact", "fast"])

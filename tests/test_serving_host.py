@@ -146,3 +146,27 @@ def test_poisson_arrivals():
     t = poisson_arrivals(20000, 200.0, np.random.default_rng(2))
     assert np.all(np.diff(t) > 0)
     assert abs(t[-1] / 20000 - 1 / 200.0) < 0.05 / 200.0
+
+
+class KeyedProgram(FakeProgram):
+    """Stages 2 and 3 share a batch key (as the window sum's rotate-and-add
+    stages do), so requests at different stages batch together there."""
+
+    def batch_key(self, stage, start_level):
+        return ("late", start_level) if stage >= 2 else (stage, start_level)
+
+    def run_group(self, key, group):
+        self.calls.append((key, tuple((r.rid, r.stage) for r in group)))
+        for r in group:
+            r.registers.setdefault("history", []).append(r.stage)
+
+
+def test_program_batch_keys_merge_stages():
+    prog = KeyedProgram()
+    loop = ServingLoop(prog, max_batch=8, timeline=FakeTimeline())
+    reqs = [request(0, 0.0), request(1, 1.5)]
+    clock, sleep = unit_clock()
+    stats = loop.run(reqs, clock=clock, sleep=sleep)
+    assert stats.completed == 2 and all(r.output == [0, 1, 2, 3] for r in reqs)
+    mixed = [members for key, members in prog.calls if key[0] == "late" and len({s for _, s in members}) > 1]
+    assert mixed, prog.calls
