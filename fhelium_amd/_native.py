@@ -68,6 +68,7 @@ SIGNATURES: dict[str, list] = {
         fh_view3, fh_view3, _i64, _i64, _i64, fh_vec, fh_table2, fh_vec, fh_vec, fh_vec, fh_vec, _ptr,
     ],
     "fh_rns_basis_extend": [fh_view3, fh_view3, _i64, _i64, _i64, _i64, fh_table2, fh_table2, _ptr],
+    "fh_rns_basis_extend_mfma": [fh_view3, fh_view3, _i64, _i64, _i64, _i64, fh_table2, fh_table2, _ptr],
     "fh_ckks_rescale": [fh_view3, fh_view3, fh_view3, fh_vec, fh_table2, _i64, _i64, _i64, _i64, _i32, _ptr],
     "fh_ckks_galois_coefficient": [fh_view3, fh_view3, _ptr, _i64, _ptr, _i64, fh_vec, _i64, _i64, _i64, _ptr],
     "fh_ckks_galois_ntt": [fh_view3, fh_view3, _ptr, _i64, _i64, _i64, _i64, _ptr],
@@ -75,6 +76,8 @@ SIGNATURES: dict[str, list] = {
     "fh_ckks_moddown": [fh_view3, fh_view3, fh_view3, fh_table2, fh_table2, _i64, _i64, _i64, _i64, _ptr],
     "fh_ckks_add_plaintext": [_i32, fh_view3, fh_view3, fh_view3, _ptr, _i64, fh_table2, _i64, _i64, _i64, _i64, _ptr],
     "fh_ckks_tensor_product": [fh_view3] * 7 + [fh_table2, _i64, _i64, _i64, _ptr],
+    "fh_ckks_plaintext_inner_product": [_ptr, _i64, _i64, _ptr, _i64, _i64, _i64, _ptr, _i64, _i64, _i64, _i64,
+                                        _i64, _i64, fh_table2, _ptr],
     "fh_ckks_keyswitch_accumulate_multi_key": [fh_view3, fh_view3, fh_view3, _ptr, _i64, _i64, _i64, _i64,
                                                fh_table2, _i64, _i64, _i64, _i64, _ptr],
     "fh_keyswitch_fused": [fh_view3, _i64, _i64, _i64, _i64, _i64, _i64, _ptr, _i64, _ptr, _ptr, fh_table2, _ptr,

@@ -222,6 +222,35 @@ __global__ __launch_bounds__(kThreads) void k_tensor_product(View3 d0, View3 d1,
   }
 }
 
+// Diagonal-method inner product: out[c, l, i] = sum_b ct[c, b, l, i] * pt[b, l, i]
+// (Montgomery products, canonical sums).  Each product is multiply_plaintext's
+// canonical value and the sum of canonical residues is order-independent, so
+// the result equals multiply_plaintext followed by pairwise additions
+// (packed_matrix_vector.py:120-170), in one read of both operands.
+__global__ __launch_bounds__(kThreads) void k_plaintext_inner_product(
+    int64_t* out, int64_t out_comp_stride, int64_t out_limb_stride, const int64_t* ct, int64_t ct_comp_stride,
+    int64_t ct_batch_stride, int64_t ct_limb_stride, const int64_t* pt, int64_t pt_batch_stride,
+    int64_t pt_limb_stride, int64_t batch, ParamView params, int64_t n) {
+  const int64_t l = blockIdx.y;
+  const int64_t c = blockIdx.z;
+  const Mont m = load_mont(params, l);
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kChunk + threadIdx.x;
+  const int64_t* crow = ct + c * ct_comp_stride + l * ct_limb_stride;
+  const int64_t* prow = pt + l * pt_limb_stride;
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) {
+    const int64_t i = base + e * kThreads;
+    if (i >= n) break;
+    const int64_t q = static_cast<int64_t>(m.q);
+    int64_t acc = 0;
+    for (int64_t b = 0; b < batch; ++b) {
+      const int64_t v = acc + mont_mul(crow[b * ct_batch_stride + i], prow[b * pt_batch_stride + i], m);
+      acc = v >= q ? v - q : v;  // both terms canonical
+    }
+    out[c * out_comp_stride + l * out_limb_stride + i] = acc;
+  }
+}
+
 int check_grid(int64_t batch, int64_t limbs, int64_t n) {
   FH_CHECK_ARG(batch >= 0 && limbs >= 0 && n >= 0, "negative extent");
   FH_CHECK_ARG(batch <= 65535 && limbs <= 65535,
@@ -234,6 +263,27 @@ int check_grid(int64_t batch, int64_t limbs, int64_t n) {
 }  // namespace fh
 
 using namespace fh;
+
+// See include/fhelium_amd.h.
+FH_API int fh_ckks_plaintext_inner_product(int64_t* out, int64_t out_comp_stride, int64_t out_limb_stride,
+                                           const int64_t* ciphertext, int64_t ct_comp_stride,
+                                           int64_t ct_batch_stride, int64_t ct_limb_stride,
+                                           const int64_t* plaintext, int64_t pt_batch_stride,
+                                           int64_t pt_limb_stride, int64_t components, int64_t batch,
+                                           int64_t limbs, int64_t n, fh_table2 rns_params, void* stream) {
+  int st = check_grid(components, limbs, n);
+  if (st != kOk) return st;
+  FH_CHECK_ARG(batch >= 1, "inner product needs at least one entry");
+  if (components == 0 || limbs == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(out && ciphertext && plaintext && rns_params.data, "inner product: null operand");
+  ParamView pv{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  dim3 grid(ceil_div(n, kChunk), limbs, components);
+  hipLaunchKernelGGL(k_plaintext_inner_product, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), out,
+                     out_comp_stride, out_limb_stride, ciphertext, ct_comp_stride, ct_batch_stride, ct_limb_stride,
+                     plaintext, pt_batch_stride, pt_limb_stride, batch, pv, n);
+  FH_CHECK_LAUNCH("plaintext inner product");
+  return kOk;
+}
 
 FH_API int fh_ckks_rescale(fh_view3 out, fh_view3 remaining, fh_view3 dropped,
                            fh_vec drop_prime_inverse_mont, fh_table2 rns_params,

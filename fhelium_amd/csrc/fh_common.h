@@ -122,6 +122,18 @@ __device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m) { return csub(x
 // REDC with R = 2^62 followed by one conditional subtraction, i.e. the
 // reference's `multiply` (montgomery.h:125-143) bit for bit, including the
 // 128-bit wrap-around of its unsigned __int128 intermediate.
+// REDC of the 128-bit product (hi, lo) with R = 2^62, then one conditional
+// subtraction (the tail of the reference's `multiply`).
+__device__ __forceinline__ int64_t mont_redc128(uint64_t lo, uint64_t hi, uint64_t q, uint64_t k) {
+  const uint64_t m = (lo * k) & kRadixMask;
+  const uint64_t mq_lo = m * q;
+  const uint64_t mq_hi = __umul64hi(m, q);
+  const uint64_t s_lo = lo + mq_lo;
+  const uint64_t s_hi = hi + mq_hi + (s_lo < lo ? 1u : 0u);
+  const uint64_t r = (s_lo >> kRadixBits) | (s_hi << (64 - kRadixBits));
+  return static_cast<int64_t>(csub(r, q));
+}
+
 __device__ __forceinline__ int64_t mont_mul(int64_t lhs, int64_t rhs,
                                             uint64_t q, uint64_t k) {
   const uint64_t a = static_cast<uint64_t>(lhs);

@@ -703,6 +703,8 @@ class CkksEngine:
         steps = [RotationKey.canonical_step(s, self.config.N) for s in rotation_steps]
         nonzero = [s for s in steps if s != 0]
         lq, n = ct.data.shape[-2:]
+        if self.fused_keyswitch and nonzero and len(nonzero) == len(steps):
+            return self._rotate_hoisted_fused(ct, [self.rotation_key(s) for s in steps])
         out = torch.empty((2, len(steps), lq, n), dtype=torch.int64, device=self.device)
         if nonzero:
             keys = [self.rotation_key(s) for s in nonzero]

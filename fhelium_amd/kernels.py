@@ -236,7 +236,10 @@ def mixed_radix_decompose(source_residues, mixed_radix_normalizers,
 
 
 def mixed_radix_basis_extend_to_montgomery(mixed_radix_components, basis_extension_coefficients,
-                                           rns_params, destination_row_count):
+                                           rns_params, destination_row_count, *, matrix_cores: bool = False):
+    """``matrix_cores``: run the byte-sliced INT8 MFMA trial kernel
+    (fh_rns_basis_extend_mfma, 1..4 digits, N a multiple of 128) instead of
+    the scalar one; the output is bit-identical."""
     op = "mixed_radix_basis_extend_to_montgomery"
     if destination_row_count <= 0:
         raise RuntimeError("destination_row_count must be positive")
@@ -251,9 +254,9 @@ def mixed_radix_basis_extend_to_montgomery(mixed_radix_components, basis_extensi
     out = torch.empty(shape, dtype=torch.int64, device=mixed_radix_components.device)
     o = nat.batch_view(out, "out")
     _check_params(op, destination_row_count, rns_params)
-    nat.call("fh_rns_basis_extend", nat.v3(o), nat.v3(digits), digits.size(0), digits.size(1),
-             destination_row_count, digits.size(2), nat.t2(c if digits.size(1) > 1 else None),
-             nat.t2(rns_params), nat.stream_ptr(out.device))
+    nat.call("fh_rns_basis_extend_mfma" if matrix_cores else "fh_rns_basis_extend", nat.v3(o), nat.v3(digits),
+             digits.size(0), digits.size(1), destination_row_count, digits.size(2),
+             nat.t2(c if digits.size(1) > 1 else None), nat.t2(rns_params), nat.stream_ptr(out.device))
     return out
 
 
@@ -514,3 +517,24 @@ def rescale_components(data: torch.Tensor, inverse: torch.Tensor, remaining_para
 
     return rescale_drop_leading_prime_nearest(data[..., 1:, :], inverse, data[..., 0, :],
                                               remaining_params, half_drop_prime)
+
+
+def plaintext_inner_product(ciphertexts: torch.Tensor, plaintexts: torch.Tensor, rns_params: torch.Tensor) -> torch.Tensor:
+    """[C, B, limb, N] NTT-domain ciphertext entries and [B, limb, N]
+    NTT-domain plaintexts -> [C, limb, N]: sum_b ct_b * pt_b (Montgomery
+    products, canonical sum) in one pass (fh_ckks_plaintext_inner_product)."""
+
+    op = "ckks_plaintext_inner_product"
+    _peers(op, ciphertexts, plaintexts, rns_params)
+    if ciphertexts.dim() != 4 or plaintexts.dim() != 3 or plaintexts.shape != ciphertexts.shape[1:]:
+        raise RuntimeError(f"{op}: expects [C, B, limb, N] ciphertexts and [B, limb, N] plaintexts")
+    if ciphertexts.stride(-1) != 1 or plaintexts.stride(-1) != 1:
+        raise RuntimeError(f"{op}: operands need a unit coefficient stride")
+    c, b, limbs, n = ciphertexts.shape
+    _check_params(op, limbs, rns_params)
+    out = torch.empty((c, limbs, n), dtype=torch.int64, device=ciphertexts.device)
+    nat.call("fh_ckks_plaintext_inner_product", out.data_ptr(), out.stride(0), out.stride(1), ciphertexts.data_ptr(),
+             ciphertexts.stride(0), ciphertexts.stride(1), ciphertexts.stride(2), plaintexts.data_ptr(),
+             plaintexts.stride(0), plaintexts.stride(1), c, b, limbs, n, nat.t2(rns_params),
+             nat.stream_ptr(ciphertexts.device))
+    return out

@@ -168,6 +168,15 @@ int fh_rns_basis_extend(fh_view3 out, fh_view3 digits, int64_t batch, int64_t di
                         int64_t destination_rows, int64_t n, fh_table2 coefficients,
                         fh_table2 rns_params, void* stream);
 
+/* mixed_radix_basis_extend_to_montgomery with the 64x64-bit products on the
+ * matrix cores (byte-sliced INT8 MFMA, v_mfma_i32_32x32x16_i8, exact int32
+ * byte-diagonal sums, then the same REDC and lazy additions): the trial of
+ * MFMA for the path's only contraction.  Bit-identical to
+ * fh_rns_basis_extend; 1..4 digits, N a multiple of 128. */
+int fh_rns_basis_extend_mfma(fh_view3 out, fh_view3 digits, int64_t batch, int64_t digit_count,
+                             int64_t destination_rows, int64_t n, fh_table2 coefficients,
+                             fh_table2 rns_params, void* stream);
+
 /* ---- CKKS (csrc/ops/ckks/ckks.cpp:29-90) ---- */
 /* rescale_drop_leading_prime_{nearest,truncate}(_): dropped is
  * [batch, 1, N] (limb stride 0). */
@@ -399,6 +408,18 @@ int fh_ckks_add_plaintext(int32_t layout, fh_view3 out, fh_view3 ciphertext, fh_
                           const int64_t* implicit_plaintext, int64_t plaintext_batch_count,
                           fh_table2 rns_params, int64_t batch, int64_t limbs, int64_t n,
                           int64_t support, void* stream);
+/* The diagonal method's inner product over a batch of ciphertext entries:
+ * out[c, l, i] = sum_b ct[c, b, l, i] * pt[b, l, i] for c < components, the
+ * Montgomery products of multiply_plaintext (ckks_engine.py:2077-2168)
+ * summed canonically -- the value multiply_plaintext followed by pairwise
+ * additions gives (fhelium/benchmarks/standalone/packed_matrix_vector.py:
+ * 120-170), in one pass.  No reference operator; the engine's matrix-vector
+ * product uses it.  Coefficient stride 1 everywhere. */
+int fh_ckks_plaintext_inner_product(int64_t* out, int64_t out_comp_stride, int64_t out_limb_stride,
+                                    const int64_t* ciphertext, int64_t ct_comp_stride, int64_t ct_batch_stride,
+                                    int64_t ct_limb_stride, const int64_t* plaintext, int64_t pt_batch_stride,
+                                    int64_t pt_limb_stride, int64_t components, int64_t batch, int64_t limbs,
+                                    int64_t n, fh_table2 rns_params, void* stream);
 /* CkksEngine.multiply's five native calls as one pass
  * (fhelium/engine/ckks_engine.py:1583-1598). */
 int fh_ckks_tensor_product(fh_view3 d0, fh_view3 d1, fh_view3 d2, fh_view3 a0, fh_view3 a1,
