@@ -17,7 +17,7 @@ import threading
 import torch
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "_lib" / "libfhelium_amd.so"
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 
 class NativeExtensionError(RuntimeError):
@@ -76,6 +76,8 @@ SIGNATURES: dict[str, list] = {
     "fh_ckks_moddown": [fh_view3, fh_view3, fh_view3, fh_table2, fh_table2, _i64, _i64, _i64, _i64, _ptr],
     "fh_ckks_add_plaintext": [_i32, fh_view3, fh_view3, fh_view3, _ptr, _i64, fh_table2, _i64, _i64, _i64, _i64, _ptr],
     "fh_ckks_tensor_product": [fh_view3] * 7 + [fh_table2, _i64, _i64, _i64, _ptr],
+    "fh_ckks_scalar_combination": [_ptr, _i64, _i64, _i64, _ptr, _ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _i64,
+                                   fh_table2, _ptr],
     "fh_ckks_plaintext_inner_product": [_ptr, _i64, _i64, _ptr, _i64, _i64, _i64, _ptr, _i64, _i64, _i64, _i64,
                                         _i64, _i64, fh_table2, _ptr],
     "fh_ckks_keyswitch_accumulate_multi_key": [fh_view3, fh_view3, fh_view3, _ptr, _i64, _i64, _i64, _i64,

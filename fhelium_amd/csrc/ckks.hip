@@ -251,6 +251,50 @@ __global__ __launch_bounds__(kThreads) void k_plaintext_inner_product(
   }
 }
 
+// Integer linear combination: out[c, e, l, i] = sum_t x_t[c, e, l, i] m_t mod q_l
+// for up to kMaxCombinationTerms terms, with s_t[l] = m_t R mod q_l so that
+// each REDC product is the canonical x m_t mod q_l.  A constant plaintext
+// (every slot the same value) encodes as the constant polynomial m, so each
+// product is what multiply_plaintext gives for it -- in either domain, since
+// the NTT of a constant polynomial is that constant -- and terms carrying
+// the same scale sum before one rescale.  Operands canonical; each term has
+// its own strides, so a term at a lower level is read from its row offset.
+constexpr int kMaxCombinationTerms = 8;
+
+struct CombinationTerms {
+  const int64_t* ptr[kMaxCombinationTerms];
+  int64_t comp_stride[kMaxCombinationTerms];
+  int64_t batch_stride[kMaxCombinationTerms];
+  int64_t limb_stride[kMaxCombinationTerms];
+  int count;
+};
+
+__global__ __launch_bounds__(kThreads) void k_scalar_combination(
+    int64_t* out, int64_t out_comp_stride, int64_t out_batch_stride, int64_t out_limb_stride,
+    CombinationTerms terms, const int64_t* scalars, int64_t scalar_stride, int64_t batch, ParamView params,
+    int64_t n) {
+  const int64_t l = blockIdx.y;
+  const int64_t c = blockIdx.z / batch;
+  const int64_t e = blockIdx.z % batch;
+  const Mont m = load_mont(params, l);
+  const int64_t q = static_cast<int64_t>(m.q);
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kChunk + threadIdx.x;
+  int64_t* orow = out + c * out_comp_stride + e * out_batch_stride + l * out_limb_stride;
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const int64_t i = base + k * kThreads;
+    if (i >= n) break;
+    int64_t acc = 0;
+    for (int t = 0; t < terms.count; ++t) {
+      const int64_t x = terms.ptr[t][c * terms.comp_stride[t] + e * terms.batch_stride[t] +
+                                     l * terms.limb_stride[t] + i];
+      const int64_t v = acc + mont_mul(x, scalars[t * scalar_stride + l], m);
+      acc = v >= q ? v - q : v;  // both terms canonical
+    }
+    orow[i] = acc;
+  }
+}
+
 int check_grid(int64_t batch, int64_t limbs, int64_t n) {
   FH_CHECK_ARG(batch >= 0 && limbs >= 0 && n >= 0, "negative extent");
   FH_CHECK_ARG(batch <= 65535 && limbs <= 65535,
@@ -282,6 +326,38 @@ FH_API int fh_ckks_plaintext_inner_product(int64_t* out, int64_t out_comp_stride
                      out_comp_stride, out_limb_stride, ciphertext, ct_comp_stride, ct_batch_stride, ct_limb_stride,
                      plaintext, pt_batch_stride, pt_limb_stride, batch, pv, n);
   FH_CHECK_LAUNCH("plaintext inner product");
+  return kOk;
+}
+
+// See include/fhelium_amd.h.
+FH_API int fh_ckks_scalar_combination(int64_t* out, int64_t out_comp_stride, int64_t out_batch_stride,
+                                      int64_t out_limb_stride, const int64_t* const* terms,
+                                      const int64_t* term_strides, const int64_t* scalars, int64_t scalar_stride,
+                                      int64_t term_count, int64_t components, int64_t batch, int64_t limbs,
+                                      int64_t n, fh_table2 rns_params, void* stream) {
+  FH_CHECK_ARG(term_count >= 1 && term_count <= kMaxCombinationTerms, "scalar combination takes 1..%d terms",
+               kMaxCombinationTerms);
+  FH_CHECK_ARG(components >= 0 && batch >= 1 && components * batch <= 65535,
+               "scalar combination: bad component (%lld) or batch (%lld) extent", (long long)components,
+               (long long)batch);
+  int st = check_grid(components * batch, limbs, n);
+  if (st != kOk) return st;
+  if (components == 0 || limbs == 0 || n == 0) return kOk;
+  FH_CHECK_ARG(out && terms && term_strides && scalars && rns_params.data, "scalar combination: null operand");
+  CombinationTerms tv{};
+  tv.count = static_cast<int>(term_count);
+  for (int64_t t = 0; t < term_count; ++t) {
+    FH_CHECK_ARG(terms[t] != nullptr, "scalar combination: term %lld is null", (long long)t);
+    tv.ptr[t] = terms[t];
+    tv.comp_stride[t] = term_strides[3 * t];
+    tv.batch_stride[t] = term_strides[3 * t + 1];
+    tv.limb_stride[t] = term_strides[3 * t + 2];
+  }
+  ParamView pv{rns_params.data, rns_params.stride_row, rns_params.stride_col};
+  dim3 grid(ceil_div(n, kChunk), limbs, components * batch);
+  hipLaunchKernelGGL(k_scalar_combination, grid, dim3(kThreads), 0, static_cast<hipStream_t>(stream), out,
+                     out_comp_stride, out_batch_stride, out_limb_stride, tv, scalars, scalar_stride, batch, pv, n);
+  FH_CHECK_LAUNCH("scalar combination");
   return kOk;
 }
 

@@ -213,7 +213,8 @@ __device__ __forceinline__ int64_t shift_positive(int64_t v, int64_t twice_q) {
 // because the reference REDC also canonicalizes; the lazy add/sub around it
 // are reproduced exactly.  Two exact algorithms are used, chosen per prime:
 //
-//  * q < 2^46 ("small" rows, the 40/41-bit scale primes): binary64.  gfx950
+//  * q < 2^42 ("small" rows, the 30/31- and 40/41-bit scale primes):
+//    binary64 (the limit the fast transform's growth bound needs).  gfx950
 //    issues FP64 FMA at the full VALU rate, while 32x32->64 integer
 //    multiplies are quarter rate, so this path is several times cheaper.
 //    With x < 2q and w < q the product is split error-free as h + l (FMA),

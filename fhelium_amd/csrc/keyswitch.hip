@@ -524,7 +524,11 @@ __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j
 // Binary64 rows: acc += w*y reduced to (-0.55q, 0.55q), with the key's
 // Montgomery value w = k R mod q (< 2q < 2^43, exact) used as a plain integer,
 // so the sum is (sum k y) R mod q -- the inverse NTT removes that R
-// (kOutStdMac).  |acc| stays below 6q over at most 10 digits.
+// (kOutStdMac).  Each term (a digit product, or the relinearization seed
+// term) lies in (-0.53q, 0.53q), so over T = digit_count + 1 <= 17 terms
+// |acc| < 9.1q, and the first inverse butterfly's difference stays below
+// 18.1q < 2^46.2 for q < 2^42 (the binary64 prime limit the host enforces):
+// inside the exactness bound |w x| / q < 2^46.4 of f64_mulmod_signed (w < q).
 __device__ __forceinline__ void mac_f64(double& acc, double y, int64_t key, double q, double qinv) {
   acc += f64_mulmod_signed(u64_to_f64(static_cast<uint64_t>(key)), y, q, qinv);
 }
@@ -536,7 +540,8 @@ __device__ __forceinline__ void mac_f64(double& acc, double y, int64_t key, doub
 // Each tile's 255 twiddles (they do not depend on the digit) and the key
 // digit indices are staged in LDS once per block, so the digit loop issues
 // only the tile and key loads -- together, at the top of the iteration --
-// and the per-thread state is small enough for 4 waves per SIMD.
+// and the per-thread state is small enough for 3 waves per SIMD (binary64
+// rows) or 2 (integer rows).
 constexpr int kMacThreads = 32;                     // threads per tile
 constexpr int kMacBlock = 128;                      // threads per tile-pass block
 constexpr int kMacTiles = kMacBlock / kMacThreads;  // tiles per block
@@ -899,8 +904,10 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
   }
   if constexpr (!kF64) {
     // Each sum holds at most 16 products y w with y < 4q (forward outputs)
-    // and w < 2q (key words), so it is below 128 q^2 < 2^127 and its REDC
-    // value r = sum 2^-64 mod q is below 128 q^2 / 2^64 + q < 9q (q < 2^60).
+    // and w < 2q (key words), plus the seed term s c with s < 2q and c < q,
+    // so it is below 130 q^2 < 2^127.1 (it fits the 128-bit accumulator) and
+    // its REDC value r = sum 2^-64 mod q is below 130 q^2 / 2^64 + q < 9.2q
+    // (q < 2^60), inside the 16q that wide_redc62's reduction chain covers.
     // The binary64 rows' REDC radix is 2^62: 4 r is the same residue as
     // sum 2^-62.  Result in [0, 2q), the inverse butterflies' input range.
 #pragma unroll
@@ -1612,7 +1619,8 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
       }
       value = static_cast<int64_t>(f64_to_u64(v));
     } else {
-      // A prime above 2^46 (the base prime): the integer formula.
+      // A prime at or above the binary64 limit 2^42 (the base prime): the
+      // integer formula.
       const Mont m = load_mont(a.params, grow);
       // (X_P R) mod q by the reference basis-extension formula, then
       // ((x R) - (X_P R)) * P^{-1} with REDC = (x - X_P) P^{-1} mod q.

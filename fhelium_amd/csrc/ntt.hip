@@ -10,10 +10,10 @@
 //  * exact: every butterfly is computed as the reference does it (canonical
 //    product, lazy add/sub modulo 2q), so the lazy forward outputs are
 //    bit-identical to the reference CPU backend, not merely congruent.
-//  * fast (default for the engine): for primes below 2^46 the residues live
+//  * fast (default for the engine): for primes below 2^42 the residues live
 //    in signed binary64 during the transform: forward butterflies are
 //    U +/- w*L with the product reduced to (-0.52q, 0.52q) and no other
-//    reduction (|x| stays below 11q < 2^45 over 16 stages); inverse
+//    reduction (|x| stays below 11q < 2^45.5 over 16 stages); inverse
 //    butterflies reduce U + L by one rounding step.  Outputs are canonical
 //    [0, q): the inverse outputs equal the reference's exactly (they are
 //    canonical there too), forward outputs equal the reference's lazy
@@ -30,10 +30,12 @@
 // Inside a pass each thread holds 16 elements in registers and performs four
 // radix-2 stages on them (a radix-16 round); rounds exchange data through
 // LDS.  The intermediate between the passes stays in the arithmetic's own
-// format (binary64 bits or u64).  Rows are processed in chunks whose
-// intermediate fits in the L2/Infinity-Cache hierarchy, so the second pass
-// re-reads it from cache rather than HBM.  Rings with logN < 12 (tests) use
-// one launch per stage.
+// format (binary64 bits or u64), in place in the operand.  The tile passes
+// run on a one-dimensional grid over (entry, tile group, limb) whose block
+// order is remapped so that consecutive work items share an XCD's L2; the
+// host may split a call into chunks of entries on several streams (a launch
+// schedule chosen per call).  Rings with logN < 12 (tests) use one launch
+// per stage.
 #include "fh_common.h"
 
 #include "ntt_core.h"
@@ -620,7 +622,12 @@ __device__ uint64_t shoup_companion(uint64_t w, uint64_t q) {
   return quo;
 }
 
-constexpr uint64_t kSmallPrimeLimit = uint64_t{1} << 46;
+// Rows of primes below 2^42 run in binary64.  The fast forward transform
+// lets values grow to 11q over its 16 unreduced stages and every binary64
+// product needs |w x| / q < 2^46.4 (w < q) to be exact, so 11q must stay
+// below 2^46.4; the key switch's accumulators (below 9.1q) need the same.
+// Larger primes take the integer path, which is exact for any q < 2^62.
+constexpr uint64_t kSmallPrimeLimit = uint64_t{1} << 42;
 
 __global__ void k_prepare_rows(ParamView params, int64_t limbs, NttRow* rows) {
   const int64_t l = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -61,7 +61,7 @@ __device__ __forceinline__ int64_t center_canonical(int64_t r, uint64_t q) {
 //   fwd/inv          butterflies; mul_const for to-Montgomery scaling.
 // ---------------------------------------------------------------------------
 
-// Exact binary64 path (q < 2^46): reference lazy semantics, see fh_common.h.
+// Exact binary64 path (q < 2^42): reference lazy semantics, see fh_common.h.
 struct F64Exact {
   using V = double;
   struct Tw {
@@ -115,7 +115,7 @@ struct F64Exact {
   }
 };
 
-// Fast binary64 path (q < 2^46): signed values, products reduced to
+// Fast binary64 path (q < 2^42): signed values, products reduced to
 // (-0.52q, 0.52q) without the final correction.
 struct F64Fast {
   using V = double;
@@ -270,7 +270,7 @@ struct U64Fast {
 
 // Packed binary64 rows.  With fast arithmetic the binary64 rows of the key
 // switch's intermediates (the ModUp output and the accumulators) travel as
-// canonical residues (< 2^46) in a u32 plane and a u16 plane, 6 bytes per
+// canonical residues (< 2^42) in a u32 plane and a u16 plane, 6 bytes per
 // coefficient instead of 8.  In the [rows, N] int64 slab of one (entry,
 // digit) or (component, entry) the u32 plane starts at the slab and the u16
 // plane follows it; both fit inside the slab's first small_rows rows, so the
@@ -307,7 +307,7 @@ __device__ __forceinline__ void packed_store(const PackedInter& pk, int64_t idx,
 
 // Inverse transform launcher shared with keyswitch.hip (fast arithmetic).
 // With in_mid the input rows hold working-format values (binary64 bits for
-// rows below 2^46, u64 otherwise) instead of int64 residues.
+// rows below 2^42, u64 otherwise) instead of int64 residues.
 int inverse_internal(int64_t* data, int64_t batch, int64_t limbs, int log_n, int64_t batch_stride,
                      int64_t limb_stride, const NttRow* rows, const int64_t* tw, const int64_t* twp,
                      int64_t row_start, int output_form, bool in_mid, hipStream_t stream);

@@ -57,7 +57,7 @@ using namespace fh;
 
 FH_API const char* fh_last_error(void) { return fh::last_error(); }
 
-FH_API int fh_abi_version(void) { return 6; }
+FH_API int fh_abi_version(void) { return 7; }
 
 // Writes "<arch> cu=<n> lds=<bytes> l2=<bytes> mem=<bytes>" for the current
 // device into buf.

@@ -381,6 +381,63 @@ class CkksEngine:
             return ct
         return result
 
+    def multiply_constants_and_add(self, cts: list[Ciphertext], values: list[float], *,
+                                   product_scale: float) -> Ciphertext:
+        """sum_t values[t] * cts[t], each constant taken at scale
+        product_scale / cts[t].scale, so that every product carries
+        product_scale; one pass over the terms (fh_ckks_scalar_combination).
+
+        A constant plaintext (every slot ``v``) at scale s encodes as the
+        constant polynomial m = round(v s) (the encoder of
+        ckks_engine.py:2077-2168 gives it), and its NTT is m in every slot,
+        so each product here has the residues multiply_plaintext gives for
+        that plaintext -- in either domain and representation, which the
+        terms share.  Terms at lower levels are read at the output's level
+        (the highest among them) by dropping their leading rows, as
+        mod_switch_to_level does.  The result is not rescaled."""
+
+        if not cts or len(cts) != len(values):
+            raise ValueError("one value per ciphertext term is required")
+        if len(cts) > K.MAX_COMBINATION_TERMS:
+            raise ValueError(f"at most {K.MAX_COMBINATION_TERMS} terms per combination")
+        first = cts[0]
+        level = max(ct.level for ct in cts)
+        for ct in cts:
+            if ct.component_count != 2 or ct.batch_shape != first.batch_shape or ct.includes_p:
+                raise CkksStateError("terms must be two-component Q-basis ciphertexts of one batch shape")
+            if (ct.polynomial_domain, ct.residue_representation) != (first.polynomial_domain,
+                                                                      first.residue_representation):
+                raise CkksStateError("terms must share polynomial domain and residue representation")
+        scale = float(product_scale)
+        if not (math.isfinite(scale) and scale > 0):
+            raise InvalidScaleError(f"product scale {product_scale} is not positive and finite")
+        ints = tuple(int(round(float(v) * (scale / ct.scale))) for v, ct in zip(values, cts))
+        scalars = self._combination_scalars(level, ints)
+        terms = []
+        for ct in cts:
+            data = ct.data[..., level - ct.level:, :]
+            terms.append(data.reshape(2, -1, data.size(-2), data.size(-1)))
+        params = self.runtime.params(level, False)
+        out = K.scalar_combination(terms, scalars, params)
+        return Ciphertext(out.view(first.data.shape[:-2] + out.shape[-2:]), level, scale,
+                          first.polynomial_domain, first.residue_representation, first.modulus_basis)
+
+    def _combination_scalars(self, level: int, ints: tuple[int, ...]) -> torch.Tensor:
+        """[T, rows] Montgomery factors m_t R mod q of the rows at ``level``,
+        cached per (level, integers): uploading them is a host-to-device copy."""
+
+        cache = self.__dict__.setdefault("_combination_cache", {})
+        key = (level, ints)
+        hit = cache.get(key)
+        if hit is None:
+            moduli = [self.config.moduli[i] for i in self.layout.prime_ids(level)]
+            rows = [[(m % q) * rns.R % q for q in moduli] for m in ints]
+            if len(cache) > 1024:
+                cache.clear()
+            hit = torch.tensor(rows, dtype=torch.int64).pin_memory().to(self.device, non_blocking=True)
+            cache[key] = hit
+        return hit
+
     def add(self, lhs: Ciphertext, rhs: Ciphertext, *, inplace: bool = False) -> Ciphertext:
         """Component-wise canonical sum at equal scale and state (ckks_engine.py:1708-1745, 3352-3377)."""
 

@@ -10,6 +10,8 @@ reference's unchanged schemas.
 
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 from fhelium_amd import _native as nat
@@ -517,6 +519,38 @@ def rescale_components(data: torch.Tensor, inverse: torch.Tensor, remaining_para
 
     return rescale_drop_leading_prime_nearest(data[..., 1:, :], inverse, data[..., 0, :],
                                               remaining_params, half_drop_prime)
+
+
+MAX_COMBINATION_TERMS = 8
+
+
+def scalar_combination(terms: list[torch.Tensor], scalars: torch.Tensor, rns_params: torch.Tensor) -> torch.Tensor:
+    """Terms of equal shape [C, E, limb, N] (any strides, unit coefficient
+    stride; canonical residues) and Montgomery scalars [T, limb]
+    (m_t R mod q) -> [C, E, limb, N]: sum_t x_t * m_t mod q, one pass
+    (fh_ckks_scalar_combination).  A term at a lower level is passed as the
+    row slice at the output's level."""
+
+    op = "ckks_scalar_combination"
+    if not 1 <= len(terms) <= MAX_COMBINATION_TERMS:
+        raise RuntimeError(f"{op}: takes 1..{MAX_COMBINATION_TERMS} terms, got {len(terms)}")
+    first = terms[0]
+    _peers(op, first, *terms[1:], scalars, rns_params)
+    if first.dim() != 4 or any(t.shape != first.shape for t in terms):
+        raise RuntimeError(f"{op}: every term must be [C, E, limb, N] of one shape")
+    if any(t.stride(-1) != 1 for t in terms):
+        raise RuntimeError(f"{op}: terms need a unit coefficient stride")
+    c, e, limbs, n = first.shape
+    _check_params(op, limbs, rns_params)
+    if scalars.dim() != 2 or scalars.shape != (len(terms), limbs) or scalars.stride(1) != 1:
+        raise RuntimeError(f"{op}: scalars must be [{len(terms)}, {limbs}] with unit limb stride")
+    out = torch.empty((c, e, limbs, n), dtype=torch.int64, device=first.device)
+    ptrs = (ctypes.c_void_p * len(terms))(*[t.data_ptr() for t in terms])
+    strides = (ctypes.c_int64 * (3 * len(terms)))(*[x for t in terms for x in t.stride()[:3]])
+    nat.call("fh_ckks_scalar_combination", out.data_ptr(), out.stride(0), out.stride(1), out.stride(2),
+             ctypes.addressof(ptrs), ctypes.addressof(strides), scalars.data_ptr(), scalars.stride(0), len(terms),
+             c, e, limbs, n, nat.t2(rns_params), nat.stream_ptr(first.device))
+    return out
 
 
 def plaintext_inner_product(ciphertexts: torch.Tensor, plaintexts: torch.Tensor, rns_params: torch.Tensor) -> torch.Tensor:

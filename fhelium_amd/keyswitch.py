@@ -16,12 +16,13 @@ import torch
 
 from fhelium_amd import kernels as K
 from fhelium_amd.core import KeySwitchKey
+from fhelium_amd import rns
 from fhelium_amd.rns import F64_DIGIT_LIMIT
 from fhelium_amd.runtime import RnsRuntime
 
 
 # Primes below this bound use the binary64 kernels (matches the NTT row records).
-SMALL_PRIME_LIMIT = 1 << 46
+SMALL_PRIME_LIMIT = rns.BINARY64_PRIME_LIMIT
 # The integer rows' lazy arithmetic (forward values below 4q, exact product
 # sums with one REDC) is sized for primes below this.
 INTEGER_PRIME_LIMIT = 1 << 60
@@ -44,7 +45,7 @@ class AccumulatorSeeds:
     """d0, d1 of an HMult front left in the NTT domain for the key switch.
 
     ``data`` is [2, *batch, Q_level, N] int64 words in a private working
-    format, not residues: rows whose prime is below 2^46 hold the binary64
+    format, not residues: rows whose prime is below 2^42 hold the binary64
     bit patterns of a b R^2 (signed, not reduced), the other rows a b R below
     2q; within each 256-coefficient tile, coefficient 8t + k sits at position
     t + 32k.  Only ``HybridKeySwitcher.key_switch_fused(seeds=...)`` reads

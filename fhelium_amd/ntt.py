@@ -9,7 +9,7 @@ contiguous interval of the canonical QP rows starting at
 The backend owns its table layout (the protocol leaves layout to backends):
 per QP row a 128-byte record (modulus, binary64 reciprocal, conversion
 constants) plus standard-form twiddles in the compact bit-reversed order
-``psi^{br(k)}``; rows of primes below 2^46 keep the twiddle as a binary64
+``psi^{br(k)}``; rows of primes below 2^42 keep the twiddle as a binary64
 bit pattern, larger primes keep ``(w, floor(w 2^64 / q))`` Shoup pairs.  The
 tables are derived on the device from the reference-format inputs
 (``rns_params`` and compact Montgomery twiddles) by ``fh_ntt_prepare_tables``.

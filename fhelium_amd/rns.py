@@ -185,6 +185,13 @@ def _as_signed64(v: int) -> int:
     return v - (1 << 64) if v >= (1 << 63) else v
 
 
+# Rows whose prime is below this limit run in binary64 arithmetic; larger
+# primes take the integer (Montgomery / Shoup) path.  2^42 is what the fast
+# transform's growth bound covers: values reach 11q over 16 unreduced
+# butterfly stages and the key-switch accumulators up to 0.53 (digits + 1) q,
+# and every binary64 product needs |w x| / q below 2^46.4 to stay exact.
+BINARY64_PRIME_LIMIT = 1 << 42
+
 # Binary64 ModDown split: a P digit c < 2^60 is taken as c_hi 2^30 + c_lo.
 MODDOWN_SPLIT_BITS = 30
 
@@ -195,7 +202,7 @@ MODDOWN_RECORD_WORDS = 16
 def moddown_binary64_table(moduli: tuple[int, ...], p_ids: tuple[int, ...],
                            q_ids: tuple[int, ...]) -> torch.Tensor:
     """[Q0, 16] float64 ModDown records, one per Q row whose prime is below
-    2^46: words 0..P-1 hold M_r mod q, words P..2P-1 hold 2^30 M_r mod q
+    BINARY64_PRIME_LIMIT: words 0..P-1 hold M_r mod q, words P..2P-1 hold 2^30 M_r mod q
     (M_r = prod_{t<r} p_t), word 2P holds P^{-1} mod q, word 14 holds q and
     word 15 holds 1/q.  Rows of larger primes are all zero (they take the
     integer path).  One 128-byte record per row lets a wave fetch a row's
@@ -211,7 +218,7 @@ def moddown_binary64_table(moduli: tuple[int, ...], p_ids: tuple[int, ...],
     out = torch.zeros((len(q_ids), MODDOWN_RECORD_WORDS), dtype=torch.float64)
     for j, qi in enumerate(q_ids):
         q = moduli[qi]
-        if q >= (1 << 46):
+        if q >= BINARY64_PRIME_LIMIT:
             continue
         for r, m in enumerate(prefix):
             out[j, r] = float(m % q)
@@ -256,7 +263,7 @@ class HostTables:
         # (their accumulators hold x R^2 and the seeds a b R^2), P R mod q for
         # the integer rows (exact sums reduced by one REDC: x R, seeds a b R).
         self.seed_scale = torch.tensor(
-            [p_product % moduli[i] if moduli[i] < (1 << 46) else (p_product * R) % moduli[i]
+            [p_product % moduli[i] if moduli[i] < BINARY64_PRIME_LIMIT else (p_product * R) % moduli[i]
              for i in self.layout.prime_ids(0)], dtype=torch.int64
         )
         self.moddown_binary64 = moddown_binary64_table(moduli, self.layout.p_prime_ids,

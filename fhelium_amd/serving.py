@@ -51,16 +51,60 @@ from fhelium_amd.core import Ciphertext, Plaintext
 # ---------------------------------------------------------------------------
 # Batched helpers over lists of unbatched ciphertexts
 # ---------------------------------------------------------------------------
-def stack(cts: Sequence[Ciphertext]) -> Ciphertext:
-    """[2, Q, N] x B -> one [2, B, Q, N] ciphertext (all states must agree)."""
-
+def _check_batchable(cts: Sequence[Ciphertext], check_scale: bool = True) -> Ciphertext:
     first = cts[0]
     for c in cts:
-        if (c.level, c.scale, c.polynomial_domain, c.residue_representation, c.modulus_basis) != (
-                first.level, first.scale, first.polynomial_domain, first.residue_representation,
-                first.modulus_basis) or c.data.shape != first.data.shape:
+        if (c.level, c.polynomial_domain, c.residue_representation, c.modulus_basis) != (
+                first.level, first.polynomial_domain, first.residue_representation,
+                first.modulus_basis) or c.data.shape != first.data.shape or (check_scale and c.scale != first.scale):
             raise ValueError("batched entries must share level, scale, state and shape")
-    data = torch.stack([c.data for c in cts], dim=1)
+    return first
+
+
+def _uniform_view(cts: Sequence[Ciphertext]) -> torch.Tensor | None:
+    """The [2, B, Q, N] view whose entries are exactly ``cts`` when they lie
+    in one storage at a constant entry stride with equal strides (entries of
+    one earlier batched result, in order), else None."""
+
+    first = cts[0].data
+    if first.dim() != 3:
+        return None
+    storage = first.untyped_storage().data_ptr()
+    step = (cts[1].data.storage_offset() - first.storage_offset()) if len(cts) > 1 else first.size(1) * first.size(2)
+    if step < first.size(1) * first.stride(1):
+        return None
+    for i, c in enumerate(cts):
+        d = c.data
+        if (d.untyped_storage().data_ptr() != storage or d.stride() != first.stride()
+                or d.storage_offset() != first.storage_offset() + i * step):
+            return None
+    st0, st1, st2 = first.stride()
+    return first.as_strided((first.size(0), len(cts), first.size(1), first.size(2)), (st0, step, st1, st2),
+                            first.storage_offset())
+
+
+def stack(cts: Sequence[Ciphertext], *, scale: float | None = None) -> Ciphertext:
+    """[2, Q, N] x B -> one [2, B, Q, N] ciphertext (all states must agree;
+    with ``scale`` given the entries may carry different scales and the
+    batch is labelled with it).  Entries that already form one contiguous
+    batched tensor (a whole earlier batched result, in order) are returned
+    as that tensor; otherwise they are copied into a new one."""
+
+    first = _check_batchable(cts, check_scale=scale is None)
+    view = _uniform_view(cts)
+    data = view if view is not None and view.is_contiguous() else torch.stack([c.data for c in cts], dim=1)
+    return Ciphertext(data, first.level, first.scale if scale is None else float(scale), first.polynomial_domain,
+                      first.residue_representation, first.modulus_basis)
+
+
+def batch_view(cts: Sequence[Ciphertext]) -> Ciphertext:
+    """Like ``stack`` but for read-only kernel operands: any constant entry
+    stride in one storage is used in place (the result may be a
+    non-contiguous view); other entries are copied."""
+
+    first = _check_batchable(cts)
+    view = _uniform_view(cts)
+    data = view if view is not None else torch.stack([c.data for c in cts], dim=1)
     return Ciphertext(data, first.level, first.scale, first.polynomial_domain, first.residue_representation,
                       first.modulus_basis)
 
@@ -87,9 +131,12 @@ class PolynomialWindowSum:
     (``chebyshev_to_power``) with the depth-optimal product tree
     y2 = y y, {y3 = y y2, y4 = y2 y2}, {y5 = y y4, y6 = y3 y3, y7 = y3 y4};
     each term a_j y^j is a plaintext-constant product whose encoding scale is
-    chosen so every term lands on the engine's default scale, and the window
-    sum is log2(window) rotate-and-add steps by 1, 2, 4, ...  A request at
-    level l ends at level l + 5.
+    chosen so that every product carries (default scale) x (the prime the
+    next rescale drops): the seven products sum in one pass and share one
+    rescale, landing on the engine's default scale.  The products of one
+    stage run as one batched HMult.  The window sum is log2(window)
+    rotate-and-add steps by 1, 2, 4, ...  A request at level l ends at
+    level l + 5.
 
     Stages (one iteration each): 0 scale by 1/radius; 1 y2; 2 y3, y4;
     3 y5, y6, y7; 4 the seven terms and a_0; 5 .. 5 + log2(window) - 1 the
@@ -136,26 +183,17 @@ class PolynomialWindowSum:
 
     def footprint_bytes(self, level: int) -> int:
         """HBM estimate of one active request at start level ``level``: ten
-        live ciphertexts and its share of a key switch's workspaces."""
+        live ciphertexts, plus its three entries of a stage's batched HMult
+        (operand copies, key-switch workspaces and outputs)."""
 
         cfg = self.engine.config
         limb = cfg.N * 8
         lq = cfg.num_q_primes - level
         lqp = lq + cfg.num_p_primes
         dnum = len(self.engine.layout.digit_specs(level))
-        return limb * (10 * 2 * lq + lq + dnum * lqp + 2 * lqp + 2 * lq)
+        return limb * (10 * 2 * lq + 3 * (2 * 2 * lq + lq + dnum * lqp + 2 * lqp + 2 * lq))
 
     # -- constants -----------------------------------------------------------
-    def _constant(self, value: float, level: int, scale: float) -> Plaintext:
-        key = (float(value), level, float(scale))
-        hit = self._plaintexts.get(key)
-        if hit is None:
-            eng = self.engine
-            hit = eng.encode(np.full(eng.config.slots, value), level=level, scale=scale,
-                             polynomial_domain="ntt")
-            self._plaintexts[key] = hit
-        return hit
-
     def _dropped_prime(self, level: int) -> int:
         return self.engine.config.moduli[self.engine.runtime.row_start(level)]
 
@@ -164,19 +202,31 @@ class PolynomialWindowSum:
         """Batched HMult + relinearize + rescale: the engine's fused HMult
         (bit-identical to NTT, multiply, relinearize, rescale in turn)."""
 
-        a = stack(lhs)
-        b = a if rhs is None else stack(rhs)
-        return unstack(self.engine.hmult(a, b))
+        # The residues of a product do not depend on the scales, so entries
+        # of different scales share one batch (labelled 1.0) and each output
+        # takes lhs.scale * rhs.scale / q, the scale the engine gives it.
+        rhs = lhs if rhs is None else rhs
+        a = stack(lhs, scale=1.0)
+        b = a if rhs is lhs else stack(rhs, scale=1.0)
+        outs = unstack(self.engine.hmult(a, b))
+        q = float(self._dropped_prime(lhs[0].level))
+        for o, x, y in zip(outs, lhs, rhs):
+            o.scale = (x.scale * y.scale) / q
+        return outs
 
-    def _const_mult(self, cts: list[Ciphertext], value: float, target_scale: float) -> Ciphertext:
-        """Batched value * ct, rescaled, landing exactly on ``target_scale``."""
+    def _scaled_sum(self, terms: list[list[Ciphertext]], values: list[float], target_scale: float) -> Ciphertext:
+        """Batched sum_t values[t] * terms[t], rescaled once, landing exactly
+        on ``target_scale``.  Each constant is the plaintext constant whose
+        encoding scale puts its product at target_scale * q (q the prime the
+        rescale drops), so the products share a scale and sum before the
+        rescale; the terms are read at the highest level among them
+        (engine.multiply_constants_and_add: coefficient domain, no NTT)."""
 
         eng = self.engine
-        batch = stack(cts)
-        q = self._dropped_prime(batch.level)
-        pt = self._constant(value, batch.level, target_scale * q / batch.scale)
-        prod = eng.multiply_plaintext(eng.coefficient_domain_to_ntt_domain(batch), pt)
-        out = eng.rescale_to_next_level(eng.ntt_domain_to_coefficient_domain(prod))
+        batches = [batch_view(t) for t in terms]
+        q = self._dropped_prime(max(b.level for b in batches))
+        acc = eng.multiply_constants_and_add(batches, values, product_scale=target_scale * q)
+        out = eng.rescale_to_next_level(acc)
         return eng.reinterpret_at_scale(out, target_scale, max_relative_change=1e-12)
 
     def _at(self, cts: list[Ciphertext], level: int) -> list[Ciphertext]:
@@ -209,8 +259,9 @@ class PolynomialWindowSum:
         eng = self.engine
         regs = [r.registers for r in group]
         delta = eng.config.default_scale
+        nreq = len(regs)
         if stage == 0:
-            y = self._const_mult([g["x"] for g in regs], 1.0 / self.radius, delta)
+            y = self._scaled_sum([[g["x"] for g in regs]], [1.0 / self.radius], delta)
             for g, v in zip(regs, unstack(y)):
                 g["y1"] = v
                 del g["x"]
@@ -218,28 +269,27 @@ class PolynomialWindowSum:
             for g, v in zip(regs, self._hmult([g["y1"] for g in regs], None)):
                 g["y2"] = v
         elif stage == 2:
+            # y3 = y1 y2 and y4 = y2 y2 as one batched HMult of 2B entries
             lvl = regs[0]["y2"].level
-            y3 = self._hmult(self._at([g["y1"] for g in regs], lvl), [g["y2"] for g in regs])
-            y4 = self._hmult([g["y2"] for g in regs], None)
-            for g, a, b in zip(regs, y3, y4):
+            y1 = self._at([g["y1"] for g in regs], lvl)
+            y2 = [g["y2"] for g in regs]
+            out = self._hmult(y1 + y2, y2 + y2)
+            for g, a, b in zip(regs, out[:nreq], out[nreq:]):
                 g["y3"], g["y4"] = a, b
         elif stage == 3:
+            # y5 = y1 y4, y6 = y3 y3 and y7 = y3 y4 as one HMult of 3B entries
             lvl = regs[0]["y4"].level
-            y5 = self._hmult(self._at([g["y1"] for g in regs], lvl), [g["y4"] for g in regs])
-            y6 = self._hmult([g["y3"] for g in regs], None)
-            y7 = self._hmult([g["y3"] for g in regs], [g["y4"] for g in regs])
-            for g, a, b, c in zip(regs, y5, y6, y7):
+            y1 = self._at([g["y1"] for g in regs], lvl)
+            y3 = [g["y3"] for g in regs]
+            y4 = [g["y4"] for g in regs]
+            out = self._hmult(y1 + y3 + y3, y4 + y3 + y4)
+            for g, a, b, c in zip(regs, out[:nreq], out[nreq: 2 * nreq], out[2 * nreq:]):
                 g["y5"], g["y6"], g["y7"] = a, b, c
         elif stage == 4:
-            lvl = regs[0]["y7"].level
-            acc = None
-            for j in range(1, 8):
-                if self.power[j] == 0.0:
-                    continue
-                term = self._const_mult(self._at([g[f"y{j}"] for g in regs], lvl), self.power[j], delta)
-                acc = term if acc is None else eng.add(acc, term, inplace=True)
-            if acc is None:
+            js = [j for j in range(1, 8) if self.power[j] != 0.0]
+            if not js:
                 raise ValueError("the polynomial has no non-constant term")
+            acc = self._scaled_sum([[g[f"y{j}"] for g in regs] for j in js], [self.power[j] for j in js], delta)
             acc = eng.add_plaintext(acc, self._constant_coefficient(self.power[0], acc.level, delta),
                                     inplace=True)
             for g, v in zip(regs, unstack(acc)):

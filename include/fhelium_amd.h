@@ -217,7 +217,7 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
  * replaces the coefficient automorphism of a rotation (ckks_engine.py:3822-3897).
  *   component        [batch, lq, N] canonical standard Q rows at `level`
  *   small_rows       number of leading QP rows at `level` whose prime is
- *                    below 2^46 (binary64 arithmetic; the rest are integer
+ *                    below 2^42 (binary64 arithmetic; the rest are integer
  *                    rows, whose primes must be below 2^60)
  *   digit_table      [digit_count, 16] int64 digit descriptors (row_begin,
  *                    rows, key_digit, addresses of the digit's mixed-radix
@@ -248,7 +248,7 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
  *                    the accumulators' scaling (what
  *                    fh_ckks_hmult_tensor_coefficient leaves in d0, d1 with
  *                    ntt_d01 = 1), with seed_scale [Q0] (P mod q for rows
- *                    below 2^46, P 2^62 mod q for the others): seed_scale x
+ *                    below 2^42, P 2^62 mod q for the others): seed_scale x
  *                    seed is added to the Q rows of accumulator 0 / 1, so the
  *                    results gain exactly d0 / d1 (floor((X + P d) / P) =
  *                    floor(X / P) + d), as base0 / base1 = d0 / d1 would. */
@@ -337,13 +337,13 @@ int fh_keyswitch_fused_static(fh_view3 component, int64_t batch, int64_t lq, int
  *   a, b             [2, batch, limbs, N] canonical standard rows at level
  *                    row_start (component, batch, limb strides given;
  *                    coefficient stride 1)
- *   small_rows       leading rows whose prime is below 2^46
+ *   small_rows       leading rows whose prime is below 2^42
  *   workspace        [4, batch, limbs, N] contiguous
  *   out              [3, batch, limbs, N] contiguous
  *   ntt_d01          0: d0, d1, d2 all in the coefficient domain; 1: d0 and
  *                    d1 stay in the NTT domain as accumulator seeds for
  *                    fh_keyswitch_fused (working format: binary64 bits of
- *                    a b R^2 for rows below 2^46, a b R as u64 below 2q for
+ *                    a b R^2 for rows below 2^42, a b R as u64 below 2q for
  *                    the others; within each 256-coefficient tile the value
  *                    of coefficient 8t + k at position t + 32k), d2 in the
  *                    coefficient domain
@@ -420,6 +420,22 @@ int fh_ckks_plaintext_inner_product(int64_t* out, int64_t out_comp_stride, int64
                                     int64_t ct_limb_stride, const int64_t* plaintext, int64_t pt_batch_stride,
                                     int64_t pt_limb_stride, int64_t components, int64_t batch, int64_t limbs,
                                     int64_t n, fh_table2 rns_params, void* stream);
+/* Integer linear combination of ciphertext components:
+ * out[c, e, l, i] = sum_t x_t[c, e, l, i] * m_t mod q_l (canonical), with
+ * scalars[t * scalar_stride + l] = m_t R mod q_l (Montgomery form).  Each
+ * product equals multiply_plaintext by a constant plaintext, which encodes
+ * as the constant polynomial m_t (ckks_engine.py:2077-2168), in either
+ * domain; same-scale terms then share one rescale.  No reference operator:
+ * the serving program's plaintext-constant products use it.  terms is a
+ * host array of term_count (1..8) device addresses of each term's row 0 at
+ * the output's level; term_strides a host array of term_count x {component,
+ * batch, limb} strides.  Coefficient stride 1 everywhere, operands
+ * canonical. */
+int fh_ckks_scalar_combination(int64_t* out, int64_t out_comp_stride, int64_t out_batch_stride,
+                               int64_t out_limb_stride, const int64_t* const* terms, const int64_t* term_strides,
+                               const int64_t* scalars, int64_t scalar_stride, int64_t term_count,
+                               int64_t components, int64_t batch, int64_t limbs, int64_t n,
+                               fh_table2 rns_params, void* stream);
 /* CkksEngine.multiply's five native calls as one pass
  * (fhelium/engine/ckks_engine.py:1583-1598). */
 int fh_ckks_tensor_product(fh_view3 d0, fh_view3 d1, fh_view3 d2, fh_view3 a0, fh_view3 a1,

@@ -59,6 +59,37 @@ def test_multiply_plaintext_then_rescale(eng):
     assert np.abs(eng.decrypt(res).real - x * y[None]).max() < 3e-5
 
 
+@pytest.mark.parametrize("level", [2, 17])
+def test_constant_products_equal_multiply_plaintext(eng, level):
+    """multiply_constants_and_add equals, bit for bit, multiply_plaintext by
+    constant plaintexts in the NTT domain (each at the encoding scale that
+    puts its product at one scale) summed with add, for terms at three
+    levels read at the highest; the combination runs in the coefficient
+    domain with no NTT."""
+    vals = [0.37, -1.25, 0.0625]
+    levels = [level, level + 1, level + 2]
+    cts = [eng.encrypt(_vals(eng, 20 + k, (2,)), level=lv) for k, lv in enumerate(levels)]
+    top = max(levels)
+    q = eng.config.moduli[eng.runtime.row_start(top)]
+    product_scale = eng.config.default_scale * q
+    got = eng.multiply_constants_and_add(cts, vals, product_scale=product_scale)
+    assert got.level == top and got.scale == product_scale
+    want = None
+    for v, ct in zip(vals, cts):
+        at = eng.mod_switch_to_level(ct, top)
+        pt = eng.encode(np.full(eng.config.slots, v), level=top, scale=product_scale / ct.scale,
+                        polynomial_domain="ntt")
+        prod = eng.ntt_domain_to_coefficient_domain(
+            eng.multiply_plaintext(eng.coefficient_domain_to_ntt_domain(at), pt))
+        prod = eng.reinterpret_at_scale(prod, product_scale, max_relative_change=1e-12)
+        want = prod if want is None else eng.add(want, prod)
+    assert torch.equal(got.data, want.data)
+    res = eng.rescale_to_next_level(got)
+    xs = [_vals(eng, 20 + k, (2,)) for k in range(3)]
+    expect = sum(v * x for v, x in zip(vals, xs))
+    assert np.abs(eng.decrypt(res).real - expect).max() < 3e-5
+
+
 def test_plaintext_state_errors(eng):
     x = _vals(eng, 5)
     ct = eng.encrypt(x, level=2)

@@ -232,3 +232,44 @@ def test_plaintext_add(setup, layout):
     else:
         got = K.add_strided_plaintext_component(dev_ct, H.to_dev(pt), H.to_dev(imp), t.rns_params.cuda())
     np.testing.assert_array_equal(H.to_np(got), O.add_plaintext(layout, ct, pt, params, imp))
+
+
+@pytest.mark.parametrize("terms", [1, 3, 8])
+def test_scalar_combination(setup, terms):
+    """fh_ckks_scalar_combination against the oracle's Montgomery row-scalar
+    products (montgomery_mul_row_scalars_canonical) summed with
+    add_canonical; terms read through row-offset and strided batch views."""
+    cfg, t, params = setup
+    rng = _rng(700 + terms)
+    m = H.moduli_array(cfg)
+    rows, start = 30, 9  # output rows 9..38
+    sub = np.ascontiguousarray(params[:, start: start + rows])
+    xs, views = [], []
+    for k in range(terms):
+        extra = k % 3  # some terms sit at a lower level: read from a row offset
+        full = H.random_residues(rng, 2 * 3, m[start - extra: start + rows], cfg.N).reshape(2, 3, rows + extra, cfg.N)
+        dev = H.to_dev(full)
+        view = dev[:, ::2][..., extra:, :] if k % 2 else dev[:, 1:3][..., extra:, :]
+        views.append(view)
+        xs.append(H.to_np(view))
+    s = np.array([[rng.integers(0, int(q)) for q in m[start: start + rows]] for _ in range(terms)], dtype=np.int64)
+    got = K.scalar_combination(views, H.to_dev(s), H.to_dev(sub))
+    assert tuple(got.shape) == (2, 2, rows, cfg.N)
+    want = None
+    for x, sc in zip(xs, s):
+        term = O.row_scalars(x.reshape(4, rows, cfg.N), sc, sub, True)
+        want = term if want is None else O.binary("add_canonical", want, term, sub)
+    np.testing.assert_array_equal(H.to_np(got).reshape(4, rows, cfg.N), want)
+
+
+def test_scalar_combination_rejects_bad_operands(setup):
+    cfg, t, params = setup
+    x = torch.zeros((2, 1, 3, cfg.N), dtype=torch.int64, device="cuda")
+    s = torch.zeros((1, 3), dtype=torch.int64, device="cuda")
+    p = H.to_dev(np.ascontiguousarray(params[:, :3]))
+    with pytest.raises(RuntimeError):
+        K.scalar_combination([x] * 9, torch.zeros((9, 3), dtype=torch.int64, device="cuda"), p)
+    with pytest.raises(RuntimeError):
+        K.scalar_combination([x, x[:, :, :2]], s, p)
+    with pytest.raises(RuntimeError):
+        K.scalar_combination([x], torch.zeros((2, 3), dtype=torch.int64, device="cuda"), p)

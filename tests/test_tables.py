@@ -92,7 +92,7 @@ def test_seed_scale_adds_p_times_the_seed(tables):
     for t, i in zip(tables.seed_scale.tolist(), tables.layout.prime_ids(0)):
         q = tables.moduli[i]
         seed = rnd.randrange(q)
-        if q < (1 << 46):
+        if q < rns.BINARY64_PRIME_LIMIT:
             added = t * seed % q  # binary64 rows: plain residue products
         else:
             added = _redc(seed, t, q)  # integer rows: one REDC per sum
