@@ -20,7 +20,10 @@ After the timed steps, a serving run (``serving`` in the JSON line) feeds
 Poisson arrivals of encrypted evaluations (a degree-7 Chebyshev polynomial
 and a 64-slot window sum) to the continuous-batching loop and reports
 throughput and p50/p99 latency, and a diagonal-method matrix-vector run
-(``matvec``: 128 diagonals at level 0) reports its time.
+(``matvec``: 128 diagonals at level 0) reports its time.  The CPU baseline
+(``cpu_baseline``, rank 0 only) times HMult + rotation at five levels on the
+host's cores: on the reference's own CPU kernels when oracle/_ref holds them
+(kind "reference"), with the oracle's C restatement beside it (kind "port").
 """
 
 from __future__ import annotations
@@ -28,7 +31,9 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import pathlib
 import statistics
+import subprocess
 import sys
 import time
 
@@ -266,13 +271,44 @@ def cpu_baseline(eng, seconds_budget: float = 20.0):
         if time.perf_counter() - t0 > seconds_budget:
             break
     dt = time.perf_counter() - t0
-    return {"value": ops / dt, "unit": "ops/s", "cores": threads, "kind": "port",
+    port = {"value": ops / dt, "unit": "ops/s", "cores": threads, "kind": "port",
             "sample": f"{rounds} round(s) of HMult + rotation at each of levels {list(levels)} "
                       "(logN=16, L34): the oracle's C restatement of the reference CPU kernels, "
                       "OpenMP over RNS rows",
-            "ops_per_s_by_level": {str(k): 2 / float(np.median(v)) for k, v in per_level.items()},
-            "os_cpu_count": os.cpu_count(), "affinity_cpus": affinity, "omp_num_threads": omp,
+            "ops_per_s_by_level": {str(k): 2 / float(np.median(v)) for k, v in per_level.items()}}
+    host = {"os_cpu_count": os.cpu_count(), "affinity_cpus": affinity, "omp_num_threads": omp,
             "torch_num_threads": torch.get_num_threads(), "cpu_model": _cpu_model()}
+    ref = reference_cpu_baseline(threads, seconds_budget, levels)
+    if ref is None:
+        return {**port, **host}
+    return {"value": ref["value"], "unit": "ops/s", "cores": threads, "kind": "reference",
+            "sample": f"{ref['rounds']} round(s) of HMult + rotation at each of levels {list(levels)} "
+                      "(logN=16, L34) on the reference's own CPU kernels (csrc/ops/*/cpu, built from "
+                      "source into oracle/_ref) composed as its engine composes them, indexed radix-2 "
+                      "NTT, ATen intra-op threads",
+            "ops_per_s_by_level": ref["ops_per_s_by_level"], "port": port, **host}
+
+
+def reference_cpu_baseline(threads: int, seconds: float, levels) -> dict | None:
+    """Times the reference's CPU kernels (oracle/_ref, when it was built) in a
+    child process: that library defines the same operator names as this
+    package's registrations.  None when it is absent or fails."""
+
+    script = pathlib.Path(__file__).resolve().parent / "oracle" / "ref_baseline.py"
+    lib = script.parent / "_ref" / "libfhelium_ref_cpu.so"
+    if not lib.exists():
+        return None
+    try:
+        proc = subprocess.run([sys.executable, str(script), "--seconds", str(seconds), "--threads", str(threads),
+                               "--levels", ",".join(str(x) for x in levels)],
+                              capture_output=True, text=True, timeout=10 * seconds + 120)
+        if proc.returncode != 0:
+            log(f"reference cpu baseline failed: {proc.stderr.strip()[-300:]}")
+            return None
+        return json.loads(proc.stdout.strip().splitlines()[-1])
+    except (subprocess.TimeoutExpired, ValueError, IndexError) as exc:
+        log(f"reference cpu baseline failed: {exc}")
+        return None
 
 
 def copy_ceiling(reps: int = 5) -> dict:
@@ -637,7 +673,7 @@ def main() -> int:
     if args.matvec_diagonals > 0:
         result["matvec"] = matvec_run(eng, args)
     if args.cpu_baseline:
-        log("cpu baseline (oracle) ...")
+        log("cpu baseline (oracle, then the reference's own CPU kernels when built) ...")
         result["cpu_baseline"] = cpu_baseline(eng)
     print(json.dumps(result), flush=True)
     return 0

@@ -45,3 +45,13 @@ def test_planted_oracle_mistake_fails_every_case():
     rc, out = _run("--plant")
     assert rc == 1
     assert len(out["mismatches"]) == out["cases"]
+
+
+def test_reference_kernel_engine_equals_oracle_engine():
+    """HMult and rotation composed on the reference's CPU kernels
+    (oracle/ref_engine.py) equal the oracle's composition (oracle/engine_ref.py)
+    bit for bit at logN=12, levels 0, 7 and 30."""
+    proc = subprocess.run([sys.executable, str(ROOT / "oracle" / "ref_baseline.py"), "--check"],
+                          capture_output=True, text=True, timeout=300, cwd=ROOT)
+    out = json.loads(proc.stdout.strip().splitlines()[-1])
+    assert proc.returncode == 0 and out["equal"], out
