@@ -1192,8 +1192,12 @@ __device__ __forceinline__ void tiles_mac_hoisted(const P& p, const KsArgs& a, i
   }
 }
 
+// Two waves per SIMD on both paths: with the next digit's loads in flight
+// the binary64 body needs 182 VGPRs; a three-wave budget spilled 44 bytes per
+// thread, and the scratch traffic cost more than the extra wave gained
+// (128-diagonal matrix-vector product 27.5 against 26.3 ms).
 template <int MODE, bool kSmall>
-__global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(kSmall ? 3 : 2))) void k_ks_mac_hoisted(
+__global__ __launch_bounds__(kMacBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_ks_mac_hoisted(
     KsArgs a, int64_t groups, int64_t j_begin, int64_t j_count) {
   __shared__ int64_t lds[mac_lds_words<kSmall>()];
   const uint32_t batch = static_cast<uint32_t>(a.batch);
