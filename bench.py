@@ -67,7 +67,7 @@ def parse_args():
     p.add_argument("--skip-plain", action="store_true", help="time only the batched mode (profiling)")
     p.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
-    p.add_argument("--serve-requests", type=int, default=300,
+    p.add_argument("--serve-requests", type=int, default=600,
                    help="requests in the serving run (0 skips it)")
     p.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     p.add_argument("--serve-inflight", type=int, default=2,
@@ -463,6 +463,7 @@ def serving_run(eng, args) -> dict:
                     "plus a 64-slot window sum (6 rotate-and-add steps); 10% at start level 16",
         "requests": stats.completed, "arrival_rate_per_s": args.serve_rate,
         "arrival_rate_realized_per_s": (n - 1) / float(arrivals[-1] - arrivals[0]) if n > 1 else None,
+        "completed_per_arrived": (stats.throughput * float(arrivals[-1] - arrivals[0]) / (n - 1)) if n > 1 else None,
         "throughput_per_s": stats.throughput, "p50_ms": stats.percentile(50) * 1e3,
         "p99_ms": stats.percentile(99) * 1e3, "max_active": stats.max_active,
         "capacity_per_s": cap.throughput, "capacity_note": "all requests arriving at once: completions per "
