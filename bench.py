@@ -70,6 +70,8 @@ def parse_args():
     p.add_argument("--serve-requests", type=int, default=600,
                    help="requests in the serving run (0 skips it)")
     p.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
+    p.add_argument("--serve-plan", default="whole", choices=["fine", "split", "whole"],
+                   help="stage plan of the serving program (steps per serving-loop iteration)")
     p.add_argument("--serve-inflight", type=int, default=2,
                    help="serving iterations the host may run ahead of the device")
     p.add_argument("--matvec-diagonals", type=int, default=128,
@@ -425,7 +427,8 @@ def serving_run(eng, args) -> dict:
                                      poisson_arrivals)
 
     rng = np.random.default_rng(args.seed + 5)
-    prog = PolynomialWindowSum(eng, logistic_chebyshev(args.seed), radius=4.0, window=64)
+    prog = PolynomialWindowSum(eng, logistic_chebyshev(args.seed), radius=4.0, window=64,
+                               stage_plan=args.serve_plan)
     reduced_level = 16
     pool = {}
     for lvl, k in ((0, 12), (reduced_level, 4)):
@@ -468,7 +471,7 @@ def serving_run(eng, args) -> dict:
         "p99_ms": stats.percentile(99) * 1e3, "max_active": stats.max_active,
         "capacity_per_s": cap.throughput, "capacity_note": "all requests arriving at once: completions per "
                                                             "second the loop sustains on one GPU",
-        "max_batch": stats.max_batch_seen, "iterations": stats.iterations,
+        "stage_plan": args.serve_plan, "max_batch": stats.max_batch_seen, "iterations": stats.iterations,
         "key_switches_per_request": 12, "max_abs_error_sample": max(errs),
         "error_bound": 64 * 2e-5,
     }
