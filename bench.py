@@ -142,70 +142,33 @@ def run_plain(eng, groups, lat_h, lat_r):
     return outs
 
 
-def group_schedule(n: int, streams: int) -> list[tuple[int, int]]:
-    """(group index, stream) in issue order.  Two streams walk the level
-    groups from both ends (the largest, lowest-level groups on one, the
-    smallest on the other, issued alternately), so a small group's short,
-    latency-bound launches run beside a large group's; more streams take the
-    groups round-robin."""
-
-    if streams <= 1:
-        return [(i, 0) for i in range(n)]
-    if streams == 2:
-        plan, lo, hi, turn = [], 0, n - 1, 0
-        while lo <= hi:
-            if turn == 0:
-                plan.append((lo, 0))
-                lo += 1
-            else:
-                plan.append((hi, 1))
-                hi -= 1
-            turn ^= 1
-        return plan
-    return [(i, i % streams) for i in range(n)]
-
-
-_STREAMS: list = []
-
-
 def run_batched(eng, groups, lat_h, lat_r, streams: int = 1):
     """One batched HMult and one batched rotation per level group.  With
     streams > 1 the independent level groups are issued on that many side
-    streams (forked from and joined back to the current stream)."""
+    streams (fhelium_amd.schedule.run_on_streams: with two, one stream walks
+    from the largest group and one from the smallest)."""
+
+    from fhelium_amd.schedule import run_on_streams
 
     items = list(groups.values())
-    outs = [None] * len(items)
-    main = torch.cuda.current_stream()
-    if streams > 1:
-        while len(_STREAMS) < streams:
-            _STREAMS.append(torch.cuda.Stream())
-        fork = torch.cuda.Event()
-        fork.record(main)
-        for st in _STREAMS[:streams]:
-            st.wait_event(fork)
-    # timing events only when latencies are recorded
-    timing = lat_h is not None
-    for i, si in group_schedule(len(items), streams):
-        g = items[i]
-        with torch.cuda.stream(_STREAMS[si] if streams > 1 else main):
-            if timing:
-                e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-                e0.record()
+
+    def task(g):
+        def run():
+            # timing events only when latencies are recorded
+            if lat_h is None:
+                return eng.rotate_by_steps(eng.hmult(g["a"], g["b"]), g["steps"])
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record()
             h = eng.hmult(g["a"], g["b"])
-            if timing:
-                e1.record()
-            outs[i] = eng.rotate_by_steps(h, g["steps"])
-            if timing:
-                e2.record()
-        if timing:
+            e1.record()
+            out = eng.rotate_by_steps(h, g["steps"])
+            e2.record()
             lat_h.append((e0, e1, len(g["steps"])))
             lat_r.append((e1, e2, len(g["steps"])))
-    if streams > 1:
-        for st in _STREAMS[:streams]:
-            join = torch.cuda.Event()
-            join.record(st)
-            main.wait_event(join)
-    return outs
+            return out
+        return run
+
+    return run_on_streams([task(g) for g in items], streams)
 
 
 def graph_leg(eng, groups, steps: int, eager_outs) -> dict:
