@@ -70,11 +70,15 @@ def run_on_streams(tasks: Sequence[Callable[[], Any]], streams: int = 2) -> list
     for st in side:
         st.wait_event(fork)
     out: list[Any] = [None] * len(tasks)
-    for i, si in issue_order(len(tasks), streams):
-        with torch.cuda.stream(side[si]):
-            out[i] = tasks[i]()
-    for st in side:
-        join = torch.cuda.Event()
-        join.record(st)
-        main.wait_event(join)
+    try:
+        for i, si in issue_order(len(tasks), streams):
+            with torch.cuda.stream(side[si]):
+                out[i] = tasks[i]()
+    finally:
+        # joined even when a task raises, so later work on the caller's
+        # stream stays ordered after everything already issued
+        for st in side:
+            join = torch.cuda.Event()
+            join.record(st)
+            main.wait_event(join)
     return out
