@@ -49,6 +49,27 @@ def test_ctypes_table_matches_header(library):
     assert set(nat.exported_symbols()) == declared
 
 
+def _declared_arity() -> dict[str, int]:
+    """Parameter count of every declared entry point ((void) counts 0)."""
+
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\b(fh_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else len(params.split(","))
+    return out
+
+
+def test_ctypes_arity_matches_header():
+    """Every ctypes signature has as many arguments as the C declaration
+    (a missing or extra argument would shift every later one)."""
+
+    arity = _declared_arity()
+    wrong = {name: (len(types), arity.get(name)) for name, types in nat.SIGNATURES.items()
+             if arity.get(name) != len(types)}
+    assert not wrong, wrong
+
+
 def test_abi_version(library):
     assert library.fh_abi_version() == nat.ABI_VERSION
 
