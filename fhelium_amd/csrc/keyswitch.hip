@@ -785,6 +785,19 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
   }
   const int64_t* tile = a.inter + ((b * a.dnum) * a.lqp + j) * n + tile_off;
   const int64_t digit_stride = a.lqp * n;
+  // Binary64 rows: software pipeline over the digits -- the next digit's
+  // packed values (one 32-byte and one 16-byte load: coefficients 8t..8t+7
+  // are contiguous in the packed block layout) load while this digit's
+  // stages and products run.
+  const uint32_t pidx = static_cast<uint32_t>(j * n + packed_index(r, 8 * t));
+  auto load_packed = [&](int64_t di, uint4& l0, uint4& l1, uint4& hh) {
+    const PackedInter pk = packed_slab(a.inter + (b * a.dnum + di) * a.lqp * n, a.small_rows, n);
+    l0 = *reinterpret_cast<const uint4*>(pk.lo + pidx);
+    l1 = *reinterpret_cast<const uint4*>(pk.lo + pidx + 4);
+    hh = *reinterpret_cast<const uint4*>(pk.hi + pidx);
+  };
+  uint4 nl0{}, nl1{}, nhh{};
+  if constexpr (std::is_same<P, F64Fast>::value) load_packed(0, nl0, nl1, nhh);
 #pragma nounroll
   for (int64_t di = 0; di < a.dnum; ++di, tile += digit_stride) {
     Global<int64_t>* k0 = key + key_digit[di] * a.key_sd + grow * a.key_sr + tile_off;
@@ -792,14 +805,9 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     V v[8];
     int64_t w0[8], w1[8];
     if constexpr (std::is_same<P, F64Fast>::value) {
-      // Eight consecutive coefficients 8t..8t+7 of the tile are contiguous
-      // in the packed block layout (one 32-byte and one 16-byte load); an
-      // LDS pass moves them to the round-1 ownership.
-      const PackedInter pk = packed_slab(a.inter + (b * a.dnum + di) * a.lqp * n, a.small_rows, n);
-      const uint32_t idx = static_cast<uint32_t>(j * n + packed_index(r, 8 * t));
-      const uint4 l0 = *reinterpret_cast<const uint4*>(pk.lo + idx);
-      const uint4 l1 = *reinterpret_cast<const uint4*>(pk.lo + idx + 4);
-      const uint4 hh = *reinterpret_cast<const uint4*>(pk.hi + idx);
+      // an LDS pass moves the values to the round-1 ownership
+      const uint4 l0 = nl0, l1 = nl1, hh = nhh;
+      if (di + 1 < a.dnum) load_packed(di + 1, nl0, nl1, nhh);
       const uint32_t lo[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
       const uint32_t hi2[4] = {hh.x, hh.y, hh.z, hh.w};
       tile_sync();  // the previous digit's last reads of the exchange buffer are done
