@@ -797,7 +797,13 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     hh = *reinterpret_cast<const uint4*>(pk.hi + pidx);
   };
   uint4 nl0{}, nl1{}, nhh{};
-  if constexpr (std::is_same<P, F64Fast>::value) load_packed(0, nl0, nl1, nhh);
+  int64_t nv[8];  // other rows: the next digit's values, likewise
+  if constexpr (std::is_same<P, F64Fast>::value) {
+    load_packed(0, nl0, nl1, nhh);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) nv[k] = tile[mac_e1(t, k)];
+  }
 #pragma nounroll
   for (int64_t di = 0; di < a.dnum; ++di, tile += digit_stride) {
     Global<int64_t>* k0 = key + key_digit[di] * a.key_sd + grow * a.key_sr + tile_off;
@@ -821,7 +827,11 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
       for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
     } else {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] = p.unmid(tile[mac_e1(t, k)]);
+      for (int k = 0; k < 8; ++k) v[k] = p.unmid(nv[k]);
+      if (di + 1 < a.dnum) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nv[k] = tile[digit_stride + mac_e1(t, k)];
+      }
     }
     // Binary64 rows issue the key loads with the tile loads; the integer
     // rows (whose products need many more registers) load them at the MAC.
