@@ -49,8 +49,8 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) per algorithmic
 # byte of the fused key switch on this workload; the rocprofv3 --pmc passes
 # and the arithmetic are recorded in profiles/keyswitch_fused_traffic.md
-# (last re-measured on the key-switch kernels of commit f68ef95).
-KS_TRAFFIC_BYTES_PER_ALG_BYTE = 2.926
+# (last re-measured with both MACs loading the next digit during the current one).
+KS_TRAFFIC_BYTES_PER_ALG_BYTE = 2.932
 # Measured VALU wave-instructions (SQ_INSTS_VALU) per algorithmic byte of the
 # fused key switch on this workload (profiles/keyswitch_fused_valu_issue.md).
 KS_VALU_INSTS_PER_ALG_BYTE = 0.33357
