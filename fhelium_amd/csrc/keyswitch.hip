@@ -325,6 +325,10 @@ __device__ __forceinline__ uint64_t extend_integer(const KsDigit& d, const NttRo
 // 2. Basis extension + forward NTT column pass.
 //    Grid (T column groups, Lqp destination rows, dnum * B).
 // ---------------------------------------------------------------------------
+template <int S1, class P>
+__device__ __forceinline__ void modup_tail(const P& p, const KsArgs& a, int64_t di, int64_t b, int64_t j, int cg,
+                                           typename P::V (&v)[16], int64_t* lds);
+
 template <int S1, int kRows, class P>
 __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const KsDigit& d,
                                               int64_t di, int64_t b, int64_t j, int cg, int64_t* lds) {
@@ -335,8 +339,6 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
   const int64_t n = int64_t{1} << a.log_n;
   const int64_t grow = a.level + j;  // global QP row of the destination
   const NttRow& row = a.rows[grow];
-  const int64_t* tw = a.fwd_w + grow * n;
-  const int64_t* twp = a.fwd_wp + grow * n;
   const int64_t* dig0 = a.digits + (b * a.lq + d.row_begin) * n;
   // kRows: the digit's row count when fixed at compile time (0: runtime)
   const int rows = kRows == 0 ? static_cast<int>(d.rows) : kRows;
@@ -415,6 +417,25 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
       for (int k = 0; k < 16; ++k) v[k] = csub(csub(v[k], 8 * q), 4 * q);
     }
   }
+  modup_tail<S1>(p, a, di, b, j, cg, v, lds);
+}
+
+// The column pass of one destination row after its basis extension: the
+// first S1 forward stages (two radix-16 rounds through LDS), then the store
+// of the intermediate (6-byte packed runs on the binary64 rows).
+template <int S1, class P>
+__device__ __forceinline__ void modup_tail(const P& p, const KsArgs& a, int64_t di, int64_t b, int64_t j, int cg,
+                                           typename P::V (&v)[16], int64_t* lds) {
+  constexpr int R1 = 1 << S1;
+  constexpr int T = R1 / 16;
+  constexpr int C = kBlock / T;
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t grow = a.level + j;
+  const int64_t* tw = a.fwd_w + grow * n;
+  const int64_t* twp = a.fwd_wp + grow * n;
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int col = cg * C + c;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int half = 8 >> s;
@@ -477,6 +498,67 @@ __device__ __forceinline__ void modup_columns(const P& p, const KsArgs& a, const
   }
 }
 
+// Binary64 rows in fast arithmetic: one block extends its (entry, digit)
+// onto two destination rows j0, j1, so every digit value it loads serves
+// both rows' sums; the two rows' column passes then run one after the
+// other through the same LDS.  Same values as modup_columns per row.
+template <int S1, int kRows>
+__device__ __forceinline__ void modup_columns_x2(const KsArgs& a, const KsDigit& d, int64_t di, int64_t b,
+                                                 int64_t j0, bool two, int cg, int64_t* lds) {
+  constexpr int R1 = 1 << S1;
+  constexpr int T = R1 / 16;
+  constexpr int C = kBlock / T;
+  const int64_t n = int64_t{1} << a.log_n;
+  const int64_t j1 = two ? j0 + 1 : j0;
+  const int64_t grow0 = a.level + j0;
+  const int64_t grow1 = a.level + j1;
+  const NttRow& row0 = a.rows[grow0];
+  const NttRow& row1 = a.rows[grow1];
+  const int64_t* dig0 = a.digits + (b * a.lq + d.row_begin) * n;
+  const int rows = kRows == 0 ? static_cast<int>(d.rows) : kRows;
+  constexpr int kMaxR = kRows == 0 ? kMaxDigitRows : kRows;
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int col = cg * C + c;
+  auto at = [&](int k) { return static_cast<uint32_t>((t + T * k) * kTile + col); };
+  double v0[16], v1[16];
+  if (d.f64_ok) {
+    const double q0 = row0.qd, qi0 = row0.qinv, q1 = row1.qd, qi1 = row1.qinv;
+    const double c00 = static_cast<double>(row0.to_mont), c10 = static_cast<double>(row1.to_mont);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const double x = __longlong_as_double(dig0[at(k)]);
+      v0[k] = f64_mulmod_signed(c00, x, q0, qi0);
+      v1[k] = f64_mulmod_signed(c10, x, q1, qi1);
+    }
+#pragma unroll
+    for (int r = 1; r < kMaxR; ++r) {
+      if (r >= rows) break;
+      const double cr0 = static_cast<double>(global(d.basis_std)[(r - 1) * d.basis_stride + grow0]);
+      const double cr1 = static_cast<double>(global(d.basis_std)[(r - 1) * d.basis_stride + grow1]);
+      const int64_t* src = dig0 + r * n;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const double x = __longlong_as_double(src[at(k)]);
+        v0[k] += f64_mulmod_signed(cr0, x, q0, qi0);
+        v1[k] += f64_mulmod_signed(cr1, x, q1, qi1);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int64_t pos = static_cast<int64_t>(t + T * k) * kTile + col;
+      v0[k] = u64_to_f64(extend_integer(d, row0, grow0, dig0, n, pos));
+      v1[k] = u64_to_f64(extend_integer(d, row1, grow1, dig0, n, pos));
+    }
+  }
+  modup_tail<S1>(F64Fast(row0), a, di, b, j0, cg, v0, lds);
+  if (two) {
+    __syncthreads();  // the first row's last reads of the LDS buffer are done
+    modup_tail<S1>(F64Fast(row1), a, di, b, j1, cg, v1, lds);
+  }
+}
+
 // One launch per arithmetic path, as for the tile pass below.
 template <int S1, int MODE, bool kSmall>
 __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j_begin, int64_t j_count) {
@@ -488,6 +570,28 @@ __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j
   constexpr uint32_t kGroups = (1u << S1) / 16;
   const uint32_t jc = static_cast<uint32_t>(j_count);
   const uint32_t dnum = static_cast<uint32_t>(a.dnum);
+  if constexpr (kSmall && MODE == 1) {
+    // work items over pairs of destination rows (the last one may be single)
+    const uint32_t pairs = (jc + 1) / 2;
+    const uint32_t w = xcd_remap(blockIdx.x, kGroups * pairs * dnum * static_cast<uint32_t>(a.batch));
+    const int cg = static_cast<int>(w % kGroups);
+    const uint32_t jp = (w / kGroups) % pairs;
+    const uint32_t z = w / (kGroups * pairs);
+    const int64_t b = z / dnum;
+    const int64_t di = z % dnum;
+    const KsDigit& d = a.dig[di];
+    const int64_t j = j_begin + 2 * static_cast<int64_t>(jp);
+    const bool two = 2 * jp + 1 < jc;
+    // the digit width is uniform over the block: a fixed-width body lets
+    // every digit load issue before the extension arithmetic
+    switch (static_cast<int>(d.rows)) {
+      case 1: modup_columns_x2<S1, 1>(a, d, di, b, j, two, cg, lds); break;
+      case 2: modup_columns_x2<S1, 2>(a, d, di, b, j, two, cg, lds); break;
+      case 4: modup_columns_x2<S1, 4>(a, d, di, b, j, two, cg, lds); break;
+      default: modup_columns_x2<S1, 0>(a, d, di, b, j, two, cg, lds); break;
+    }
+    return;
+  }
   const uint32_t w = xcd_remap(blockIdx.x, kGroups * jc * dnum * static_cast<uint32_t>(a.batch));
   const int cg = static_cast<int>(w % kGroups);
   const int64_t j = j_begin + (w / kGroups) % jc;
@@ -496,16 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_ks_modup_columns(KsArgs a, int64_t j
   const int64_t di = z % dnum;
   const KsDigit& d = a.dig[di];
   const NttRow& row = a.rows[a.level + j];
-  if constexpr (kSmall && MODE == 1) {
-    // the digit width is uniform over the block: a fixed-width body lets
-    // every digit load issue before the extension arithmetic
-    switch (static_cast<int>(d.rows)) {
-      case 1: modup_columns<S1, 1>(F64Fast(row), a, d, di, b, j, cg, lds); break;
-      case 2: modup_columns<S1, 2>(F64Fast(row), a, d, di, b, j, cg, lds); break;
-      case 4: modup_columns<S1, 4>(F64Fast(row), a, d, di, b, j, cg, lds); break;
-      default: modup_columns<S1, 0>(F64Fast(row), a, d, di, b, j, cg, lds); break;
-    }
-  } else if constexpr (kSmall) {
+  if constexpr (kSmall) {
     modup_columns<S1, 0>(F64Exact(row), a, d, di, b, j, cg, lds);
   } else {
     switch (static_cast<int>(d.rows)) {
@@ -1840,7 +1935,9 @@ int keyswitch_fused_impl(fh_view3 component, int64_t batch, int64_t lq, int64_t 
     const int64_t j_begin = pass == 0 ? 0 : small_rows;
     const int64_t j_count = pass == 0 ? small_rows : lqp - small_rows;
     if (j_count == 0) return kOk;
-    const int64_t modup_blocks = (kTile / C) * j_count * digit_count * batch_in;
+    // binary64 rows in fast arithmetic: one block per pair of rows
+    const int64_t j_items = pass == 0 && mode == 1 ? (j_count + 1) / 2 : j_count;
+    const int64_t modup_blocks = (kTile / C) * j_items * digit_count * batch_in;
     FH_CHECK_ARG(modup_blocks <= 0x7fffffff, "key switch grid too large");
     const dim3 gcol(static_cast<uint32_t>(modup_blocks));
 #define FH_MODUP_CASE(S)                                                                           \
