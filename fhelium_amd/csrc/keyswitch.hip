@@ -6,8 +6,8 @@
 // csrc/ops/ckks/cpu/ckks_cpu.cpp:474-708), reorganised into eight launches
 // (nine for a rotation) instead of ~100 (ModUp and the tile pass launch once
 // for the binary64 rows
-// and once for the integer rows; the integer rows' two launches run on a
-// side stream, concurrently with the binary64 rows' launches):
+// and once for the integer rows; the integer rows' MAC runs on a side
+// stream, concurrently with the binary64 rows' launches):
 //
 //   1. k_ks_decompose     Garner digits of every active hybrid digit (one
 //                         launch for all digits); optionally reads its input
@@ -1767,12 +1767,14 @@ __global__ __launch_bounds__(kBlock) void k_ks_moddown_add(ModDownArgs a) {
   }
 }
 
-// The integer-row (60-bit prime) ModUp and MAC launches are VALU-bound and
-// the binary64-row MAC is HBM-bound, so the two paths run concurrently: the
-// integer rows on a side stream forked from the caller's stream by an event
-// and joined back before the inverse column pass.  One side stream and one
-// fork/join event pair per device, created on first use; the mutex keeps
-// the pair to one fused call at a time.
+// The integer-row (60-bit prime) ModUp runs first on the caller's stream;
+// then the integer-row MAC runs on a side stream, forked from the caller's
+// stream by an event, beside the binary64 rows' ModUp and MAC, and is joined
+// back before the inverse column pass.  (Running both integer launches on
+// the side stream from the start, or forking after the binary64 ModUp, was
+// slower: the integer ModUp then contends with the binary64 MAC.)  One side
+// stream and one fork/join event pair per device, created on first use; the
+// mutex keeps the pair to one fused call at a time.
 struct SideStream {
   hipStream_t stream = nullptr;
   hipEvent_t fork = nullptr;
@@ -1837,8 +1839,8 @@ int scratch_address(int64_t** out) {
 }
 
 // The eight-launch fused key switch.  With use_side_stream the integer rows'
-// ModUp and MAC run on the device's side stream concurrently with the
-// binary64 rows; without it every launch goes to the caller's stream.
+// MAC runs on the device's side stream concurrently with the binary64 rows'
+// ModUp and MAC; without it every launch goes to the caller's stream.
 int keyswitch_fused_impl(fh_view3 component, int64_t batch, int64_t lq, int64_t lqp,
                          int64_t level, int64_t log_n, int64_t small_rows,
                          const int64_t* digit_table,
@@ -2007,12 +2009,13 @@ int keyswitch_fused_impl(fh_view3 component, int64_t batch, int64_t lq, int64_t 
     if (rc != kOk) return rc;
     const bool split = small_rows > 0 && small_rows < lqp;
     if (split) {
+      rc = launch_modup(1, s);
+      if (rc != kOk) return rc;
       if (hipEventRecord(side->fork, s) != hipSuccess || hipStreamWaitEvent(side->stream, side->fork, 0) != hipSuccess) {
         set_error("fused key switch: fork failed: %s", hipGetErrorString(hipGetLastError()));
         return kLaunchFailed;
       }
-      rc = launch_modup(1, side->stream);
-      if (rc == kOk) rc = launch_mac(1, side->stream);
+      rc = launch_mac(1, side->stream);
       if (rc != kOk) return rc;
       if (hipEventRecord(side->join, side->stream) != hipSuccess) {
         set_error("fused key switch: join record failed: %s", hipGetErrorString(hipGetLastError()));
