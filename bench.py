@@ -49,11 +49,11 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) per algorithmic
 # byte of the fused key switch on this workload; the rocprofv3 --pmc passes
 # and the arithmetic are recorded in profiles/keyswitch_fused_traffic.md
-# (last re-measured with both MACs loading the next digit during the current one).
-KS_TRAFFIC_BYTES_PER_ALG_BYTE = 2.932
+# (last re-measured on the current key-switch kernels).
+KS_TRAFFIC_BYTES_PER_ALG_BYTE = 2.956
 # Measured VALU wave-instructions (SQ_INSTS_VALU) per algorithmic byte of the
 # fused key switch on this workload (profiles/keyswitch_fused_valu_issue.md).
-KS_VALU_INSTS_PER_ALG_BYTE = 0.33357
+KS_VALU_INSTS_PER_ALG_BYTE = 0.33187
 # gfx950 issues one wave64 VALU instruction per CU per cycle for full-rate
 # operations (the rate behind the 78.6 TFLOP/s binary64 FMA peak):
 # 256 CUs x 2.4 GHz.
