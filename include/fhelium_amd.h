@@ -215,6 +215,10 @@ int fh_ckks_keyswitch_accumulate_multi_key(fh_view3 acc0, fh_view3 acc1, fh_view
  * fhelium/engine/hybrid_keyswitch.py:321-371 plus :530-568 and
  * ckks_engine.py:1664-1667 / :3342, bit for bit; with Galois tables it also
  * replaces the coefficient automorphism of a rotation (ckks_engine.py:3822-3897).
+ * The integer rows' MAC runs on a library-owned side stream (one per device)
+ * forked from `stream` by an event and joined back to it before the inverse
+ * transform, so the call is ordered like launches on `stream` alone (and is
+ * captured with it).
  *   component        [batch, lq, N] canonical standard Q rows at `level`
  *   small_rows       number of leading QP rows at `level` whose prime is
  *                    below 2^42 (binary64 arithmetic; the rest are integer
