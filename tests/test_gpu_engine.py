@@ -297,3 +297,34 @@ def test_fused_hoisted_rotations_bit_exact(small, level):
         np.testing.assert_array_equal(_np(many[e].data), _np(batch.data[:, e]), err_msg=f"by steps {e}")
     err = np.abs(eng.decrypt(batch).real[3] - np.roll(v, 513)).max()
     assert err < TOL_KEYSWITCH, err
+
+
+@pytest.fixture(scope="module")
+def full_parity_exact():
+    eng = CkksEngine(H.config(16), ntt_mode="exact", rng_seed=19)
+    eng.generate_secret_key()
+    eng.generate_relinearization_key()
+    return eng, EngineRef(H.host_tables(16))
+
+
+@pytest.mark.parametrize("level", [0, 21])
+def test_full_size_exact_mode_bit_exact(full_parity_exact, level):
+    """logN=16, L34, exact NTT arithmetic (the reference's lazy butterflies;
+    the key switch's exact binary64 ModUp and MAC): NTT conversion equals
+    the oracle's lazy values, and HMult (composition) and a rotation
+    reproduce the oracle's residues bit for bit."""
+
+    eng, ref = full_parity_exact
+    print(f"exact mode logN=16 level={level}", flush=True)
+    a, _ = _fresh(eng, level, batch=(1,), seed=13)
+    b, _ = _fresh(eng, level, batch=(1,), seed=14)
+    na = eng.coefficient_domain_to_ntt_domain(a)
+    ra = ref.coefficient_domain_to_ntt_domain(_np(a.data), level)
+    np.testing.assert_array_equal(_np(na.data), ra)
+    got = eng.hmult(a, b)
+    rprod = ref.multiply(ra, ref.coefficient_domain_to_ntt_domain(_np(b.data), level), level)
+    want = ref.rescale(ref.relinearize(rprod, _np(eng.relinearization_key.data), level), level)
+    np.testing.assert_array_equal(_np(got.data), want)
+    key = eng.rotation_key(-7)
+    rot = eng.rotate_by_step(got, -7)
+    np.testing.assert_array_equal(_np(rot.data), ref.rotate(want, _np(key.data), key.galois_element, level + 1))
