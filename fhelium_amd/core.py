@@ -10,6 +10,8 @@ level-``l`` Q value is prime row ``l + j``.
 
 from __future__ import annotations
 
+import math
+import operator
 from dataclasses import dataclass, field, replace
 
 import torch
@@ -32,8 +34,57 @@ class InvalidScaleError(ValueError):
     pass
 
 
+def _check_scale(scale, value_name: str) -> float:
+    """A positive finite binary64 scale (core/scale.py:11-29)."""
+
+    if isinstance(scale, (bool, str, bytes)):
+        raise InvalidScaleError(f"{value_name} scale must be a positive finite number, got {scale!r}")
+    try:
+        value = float(scale)
+    except (TypeError, ValueError, OverflowError) as error:
+        raise InvalidScaleError(f"{value_name} scale must be a positive finite number, got {scale!r}") from error
+    if not math.isfinite(value) or value <= 0.0:
+        raise InvalidScaleError(f"{value_name} scale must be a positive finite number, got {scale!r}")
+    return value
+
+
+def _check_payload(data, value_name: str, min_dims: int) -> None:
+    """Dense integral tensor (core/validation.py:62-77) with non-empty limb
+    and coefficient axes and nonzero batch extents."""
+
+    if not isinstance(data, torch.Tensor):
+        raise TypeError(f"{value_name} data must be a torch.Tensor")
+    if data.layout != torch.strided:
+        raise TypeError(f"{value_name} data must use dense strided storage")
+    if data.dtype == torch.bool or data.is_floating_point() or data.is_complex():
+        raise TypeError(f"{value_name} data must use an integral scalar dtype")
+    if data.dim() < min_dims:
+        raise ValueError(f"{value_name} data has too few axes: shape {tuple(data.shape)}")
+    if data.size(-2) == 0 or data.size(-1) == 0:
+        raise ValueError(f"{value_name} limb and coefficient axes cannot be empty")
+
+
+def _check_level(level, value_name: str) -> int:
+    if isinstance(level, bool):
+        raise ValueError(f"{value_name} level must be a non-negative integer, got {level!r}")
+    try:
+        value = operator.index(level)
+    except TypeError as error:
+        raise ValueError(f"{value_name} level must be a non-negative integer, got {level!r}") from error
+    if value < 0:
+        raise ValueError(f"{value_name} level must be a non-negative integer, got {level!r}")
+    return value
+
+
 @dataclass
 class Ciphertext:
+    """``data`` is [component, *batch, limb, coeff]; construction applies the
+    structural checks of the reference's Ciphertext
+    (fhelium/core/ciphertext.py:67-127): 2 or 3 components, non-empty limb
+    and coefficient axes, nonzero batch dimensions, a known domain, basis
+    and representation (coefficient values standard, NTT values Montgomery),
+    a non-negative level and a positive finite scale."""
+
     data: torch.Tensor
     level: int
     scale: float
@@ -41,6 +92,25 @@ class Ciphertext:
     residue_representation: str = "standard"
     modulus_basis: str = "Q"
     context_id: int = 0
+
+    def __post_init__(self) -> None:
+        self.scale = _check_scale(self.scale, "Ciphertext")
+        self.level = _check_level(self.level, "Ciphertext")
+        _check_payload(self.data, "Ciphertext", 3)
+        if self.data.size(0) not in (2, 3):
+            raise ValueError(f"Ciphertext component axis must have size 2 or 3, got {self.data.size(0)}")
+        if any(extent == 0 for extent in self.data.shape[1:-2]):
+            raise ValueError("Ciphertext batch dimensions must be nonzero")
+        if self.polynomial_domain not in ("coefficient", "ntt"):
+            raise ValueError(f"Unsupported ciphertext polynomial_domain: {self.polynomial_domain!r}")
+        if self.modulus_basis not in ("Q", "QP"):
+            raise ValueError(f"Unsupported ciphertext modulus_basis: {self.modulus_basis!r}")
+        if self.residue_representation not in ("standard", "montgomery"):
+            raise ValueError(f"Unsupported ciphertext residue representation: {self.residue_representation!r}")
+        if self.polynomial_domain == "coefficient" and self.residue_representation != "standard":
+            raise ValueError("a coefficient-domain Ciphertext holds standard residues")
+        if self.polynomial_domain == "ntt" and self.residue_representation != "montgomery":
+            raise ValueError("an NTT-domain Ciphertext holds Montgomery residues")
 
     @property
     def component_count(self) -> int:
@@ -109,6 +179,19 @@ class Plaintext:
     polynomial_domain: str = "coefficient"
     residue_representation: str = "montgomery"
     modulus_basis: str = "Q"
+
+    def __post_init__(self) -> None:
+        # the structural checks the reference applies to its values
+        # (fhelium/core/ciphertext.py:67-100, core/scale.py:11-29)
+        self.scale = _check_scale(self.scale, "Plaintext")
+        self.level = _check_level(self.level, "Plaintext")
+        _check_payload(self.data, "Plaintext", 2)
+        if any(extent == 0 for extent in self.data.shape[:-2]):
+            raise ValueError("Plaintext batch dimensions must be nonzero")
+        if self.polynomial_domain not in ("coefficient", "ntt"):
+            raise ValueError(f"Unsupported plaintext polynomial_domain: {self.polynomial_domain!r}")
+        if self.modulus_basis not in ("Q", "QP"):
+            raise ValueError(f"Unsupported plaintext modulus_basis: {self.modulus_basis!r}")
 
     @property
     def batch_shape(self) -> tuple[int, ...]:
