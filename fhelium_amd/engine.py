@@ -14,6 +14,7 @@ come from an external package that is not available here).
 from __future__ import annotations
 
 import math
+from dataclasses import replace
 
 import numpy as np
 import torch
@@ -452,21 +453,103 @@ class CkksEngine:
         K.add_canonical_(flat, rhs.data.reshape(flat.shape), params)
         return result
 
-    def ntt_domain_to_coefficient_domain(self, ct: Ciphertext) -> Ciphertext:
-        """NTT/Montgomery -> coefficient/standard canonical (ckks_engine.py:931-985)."""
+    # ------------------------------------------------------------------
+    # Domain transitions (ckks_engine.py:867-985, 2790-2835): strict source
+    # states re-checked at call time; functional forms return new storage,
+    # the in-place forms (trailing underscore) transform the payload in its
+    # storage and update the state fields.  Ciphertexts move between
+    # (coefficient, standard) and (ntt, montgomery); plaintexts keep
+    # Montgomery residues in both domains.
+    # ------------------------------------------------------------------
+    def _domain_source(self, value, source_domain: str) -> None:
+        operation = ("coefficient_domain_to_ntt_domain" if source_domain == "coefficient"
+                     else "ntt_domain_to_coefficient_domain")
+        article = "a coefficient-domain" if source_domain == "coefficient" else "an NTT-domain"
+        if isinstance(value, Plaintext):
+            kind = "plaintext"
+            if value.modulus_basis not in ("Q", "QP"):
+                raise ValueError(f"Unsupported plaintext modulus_basis: {value.modulus_basis!r}")
+            if value.polynomial_domain != source_domain:
+                raise CkksStateError(f"{operation} requires {article} {kind}")
+            if value.residue_representation != "montgomery":
+                raise CkksStateError(f"{operation} requires Montgomery plaintext residues")
+        elif isinstance(value, Ciphertext):
+            if value.modulus_basis not in ("Q", "QP"):
+                raise ValueError(f"Unsupported ciphertext modulus_basis: {value.modulus_basis!r}")
+            if value.polynomial_domain != source_domain:
+                raise CkksStateError(f"{operation} requires {article} ciphertext")
+            if source_domain == "coefficient" and value.residue_representation != "standard":
+                raise CkksStateError(f"{operation} requires standard ciphertext residues")
+            if source_domain == "ntt" and value.residue_representation != "montgomery":
+                raise CkksStateError(f"{operation} requires Montgomery ciphertext residues")
+        else:
+            raise TypeError(f"Domain transition expects Ciphertext or Plaintext, got {type(value).__name__}")
+        self.layout.check_level(value.level)
+        rows = value.data.size(-2)
+        start = self.runtime.row_start(value.level)
+        want = self.config.num_q_primes - start + (self.config.num_p_primes if value.modulus_basis == "QP" else 0)
+        if rows != want or value.data.size(-1) != self.config.N:
+            raise ValueError(f"{operation}: {rows} limbs of {value.data.size(-1)} coefficients do not match "
+                             f"level {value.level} basis {value.modulus_basis} ({want} x {self.config.N})")
 
-        ct.assert_state(polynomial_domain="ntt", residue_representation="montgomery")
+    def _forward_ntt_(self, value):
         rt = self.runtime
-        data = rt.ntt.inverse_to_standard(ct.data, rt.row_start(ct.level))
-        return Ciphertext(data, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
+        start = rt.row_start(value.level)
+        if isinstance(value, Ciphertext):
+            rt.ntt.forward_to_montgomery_(value.data, start)
+            value.residue_representation = "montgomery"
+        else:
+            rt.ntt.forward_montgomery_(value.data, start)
+        value.polynomial_domain = "ntt"
+        return value
+
+    def _inverse_ntt_(self, value):
+        rt = self.runtime
+        start = rt.row_start(value.level)
+        if isinstance(value, Ciphertext):
+            rt.ntt.inverse_to_standard_(value.data, start)
+            value.residue_representation = "standard"
+        else:
+            rt.ntt.inverse_montgomery_(value.data, start)
+        value.polynomial_domain = "coefficient"
+        return value
+
+    def coefficient_domain_to_ntt_domain_(self, value):
+        """coefficient_domain_to_ntt_domain in place (ckks_engine.py:914-928)."""
+
+        self._domain_source(value, "coefficient")
+        return self._forward_ntt_(value)
+
+    def ntt_domain_to_coefficient_domain(self, value):
+        """NTT -> coefficient domain (ckks_engine.py:940-961): a ciphertext's
+        (ntt, montgomery) becomes canonical (coefficient, standard), a
+        plaintext's (ntt, montgomery) becomes (coefficient, montgomery)."""
+
+        self._domain_source(value, "ntt")
+        if isinstance(value, Plaintext):
+            return self._inverse_ntt_(replace(value, data=value.data.clone()))
+        rt = self.runtime
+        data = rt.ntt.inverse_to_standard(value.data, rt.row_start(value.level))
+        return Ciphertext(data, value.level, value.scale, "coefficient", "standard", value.modulus_basis,
+                          value.context_id)
+
+    def ntt_domain_to_coefficient_domain_(self, value):
+        """ntt_domain_to_coefficient_domain in place (ckks_engine.py:973-985)."""
+
+        self._domain_source(value, "ntt")
+        return self._inverse_ntt_(value)
 
     # ------------------------------------------------------------------
     # The hot path
     # ------------------------------------------------------------------
-    def coefficient_domain_to_ntt_domain(self, ct: Ciphertext) -> Ciphertext:
-        """Coefficient/standard -> NTT/Montgomery (ckks_engine.py:876-903, 3094-3109)."""
+    def coefficient_domain_to_ntt_domain(self, ct):
+        """Coefficient -> NTT domain (ckks_engine.py:876-903, 3094-3109): a
+        ciphertext's (coefficient, standard) becomes (ntt, montgomery), a
+        plaintext's (coefficient, montgomery) becomes (ntt, montgomery)."""
 
-        ct.assert_state(polynomial_domain="coefficient", residue_representation="standard")
+        self._domain_source(ct, "coefficient")
+        if isinstance(ct, Plaintext):
+            return self._forward_ntt_(replace(ct, data=ct.data.clone()))
         data = self.runtime.ntt.forward_to_montgomery(ct.data, self.runtime.row_start(ct.level))
         return Ciphertext(data, ct.level, ct.scale, "ntt", "montgomery", ct.modulus_basis, ct.context_id)
 
