@@ -533,6 +533,21 @@ class CkksEngine:
         return Ciphertext(data, value.level, value.scale, "coefficient", "standard", value.modulus_basis,
                           value.context_id)
 
+    def add_(self, lhs: Ciphertext, rhs: Ciphertext) -> Ciphertext:
+        """add in place into ``lhs`` (ckks_engine.py:1752)."""
+
+        return self.add(lhs, rhs, inplace=True)
+
+    def add_plaintext_(self, ct: Ciphertext, plaintext: Plaintext) -> Ciphertext:
+        """add_plaintext in place into ``ct`` (ckks_engine.py:2068)."""
+
+        return self.add_plaintext(ct, plaintext, inplace=True)
+
+    def multiply_plaintext_(self, ct: Ciphertext, plaintext: Plaintext) -> Ciphertext:
+        """multiply_plaintext in place into ``ct`` (ckks_engine.py:2170)."""
+
+        return self.multiply_plaintext(ct, plaintext, inplace=True)
+
     def ntt_domain_to_coefficient_domain_(self, value):
         """ntt_domain_to_coefficient_domain in place (ckks_engine.py:973-985)."""
 
@@ -588,10 +603,9 @@ class CkksEngine:
             K.add_two_canonicalize(out, d[0], d[1], corr0, corr1, rt.params(ct.level))
         return Ciphertext(out, ct.level, ct.scale, "coefficient", "standard", ct.modulus_basis)
 
-    def rescale_to_next_level(self, ct: Ciphertext) -> Ciphertext:
-        """Divide-round-drop of the leading Q prime (ckks_engine.py:1131-1174,
-        ckks_rescale.py:84-131, 224-377)."""
-
+    def _rescale_tables(self, ct: Ciphertext, rounding: str):
+        if rounding not in ("nearest", "floor"):
+            raise ValueError(f"rounding must be 'nearest' or 'floor', got {rounding!r}")
         ct.assert_state(polynomial_domain="coefficient", residue_representation="standard")
         cfg = self.config
         next_level = ct.level + 1
@@ -600,12 +614,47 @@ class CkksEngine:
         rt = self.runtime
         start = rt.row_start(ct.level)
         stop = cfg.total_num_primes if ct.includes_p else cfg.num_q_primes
-        remaining = rt.params_rows(start + 1, stop)
-        inv = rt.rescale_inverses[ct.level][: stop - start - 1]
-        dropped_prime = cfg.moduli[start]
-        out = K.rescale_components(ct.data, inv, remaining, dropped_prime // 2)
-        return Ciphertext(out, next_level, ct.scale / float(dropped_prime), ct.polynomial_domain,
-                          ct.residue_representation, ct.modulus_basis)
+        if ct.data.size(-2) != stop - start:
+            raise ValueError(f"rescale: {ct.data.size(-2)} limbs do not match level {ct.level} basis "
+                             f"{ct.modulus_basis} ({stop - start})")
+        scale = ct.scale / float(cfg.moduli[start])
+        if not math.isfinite(scale) or scale <= 0:
+            raise InvalidScaleError("rescaled scale is not positive and finite")
+        return (next_level, scale, rt.params_rows(start + 1, stop), rt.rescale_inverses[ct.level][: stop - start - 1],
+                cfg.moduli[start])
+
+    def rescale_to_next_level(self, ct: Ciphertext, *, rounding: str = "nearest") -> Ciphertext:
+        """Divide-round-drop of the leading Q prime (ckks_engine.py:1131-1174,
+        ckks_rescale.py:84-131, 224-377): ``rounding="nearest"`` rounds the
+        quotient to the nearest integer, ``"floor"`` subtracts the least
+        non-negative dropped residue before the exact division."""
+
+        next_level, scale, remaining, inv, dropped_prime = self._rescale_tables(ct, rounding)
+        if rounding == "nearest":
+            out = K.rescale_components(ct.data, inv, remaining, dropped_prime // 2)
+        else:
+            out = K.rescale_drop_leading_prime_truncate(ct.data[..., 1:, :], inv, ct.data[..., 0, :], remaining)
+        return Ciphertext(out, next_level, scale, ct.polynomial_domain, ct.residue_representation, ct.modulus_basis,
+                          ct.context_id)
+
+    def rescale_to_next_level_(self, ct: Ciphertext, *, rounding: str = "nearest") -> Ciphertext:
+        """rescale_to_next_level in place (ckks_engine.py:1176-1246): the
+        remaining rows are updated in the original allocation, ``ct.data``
+        becomes the view of those rows, and level and scale advance."""
+
+        next_level, scale, remaining, inv, dropped_prime = self._rescale_tables(ct, rounding)
+        rows = ct.data[..., 1:, :]
+        # the dropped row is read while the rows after it are written: a copy
+        # of it keeps the operands apart
+        dropped = ct.data[..., 0, :].clone()
+        if rounding == "nearest":
+            K.rescale_drop_leading_prime_nearest_(rows, inv, dropped, remaining, dropped_prime // 2)
+        else:
+            K.rescale_drop_leading_prime_truncate_(rows, inv, dropped, remaining)
+        ct.data = rows
+        ct.level = next_level
+        ct.scale = scale
+        return ct
 
     def mod_switch_to_level(self, ct: Ciphertext, target_level: int) -> Ciphertext:
         """Restrict ``ct`` to the basis at ``target_level`` by dropping its
@@ -656,6 +705,29 @@ class CkksEngine:
         for hop in self.rotation_path(step):
             result = self.rotate_with_key(result, self.rotation_key(hop))
         return result
+
+    def rotate_by_step_(self, ct: Ciphertext, rotation_step: int) -> Ciphertext:
+        """``ct`` replaced by rotate_by_step's result (ckks_engine.py:2333-2346):
+        the object keeps its identity, ``ct.data`` becomes the new storage."""
+
+        result = self.rotate_by_step(ct, rotation_step)
+        ct.data = result.data
+        ct.level, ct.scale = result.level, result.scale
+        ct.polynomial_domain, ct.residue_representation = result.polynomial_domain, result.residue_representation
+        ct.modulus_basis = result.modulus_basis
+        return ct
+
+    def switch_key(self, ct: Ciphertext, key: KeySwitchKey) -> Ciphertext:
+        """c0 + c1 s_src -> c0' + c1' s_dst with a key generated for that
+        direction (ckks_engine.py:1677-1706): a two-component coefficient
+        standard Q ciphertext in, a new one out, same level and scale."""
+
+        ct.assert_state(polynomial_domain="coefficient", modulus_basis="Q", residue_representation="standard",
+                        components=2)
+        if not isinstance(key, KeySwitchKey):
+            raise TypeError(f"switch_key expects a KeySwitchKey, got {type(key).__name__}")
+        self.layout.check_level(ct.level)
+        return self._switch_rotated(ct, ct.data, key)
 
     def rotation_path(self, rotation_step: int) -> list[int]:
         """Canonical steps whose rotations, applied in order, realise
