@@ -22,10 +22,11 @@ class CkksStateError(ValueError):
 
 
 class MaximumLevelError(ValueError):
-    """Rescale requested past the last public level (errors.py, ckks_rescale.py:166-173)."""
+    """Rescale or mod switch requested past the last public level (errors.py,
+    ckks_rescale.py:166-173)."""
 
     def __init__(self, level: int, maximum_level: int) -> None:
-        super().__init__(f"cannot rescale from level {level}: the last public level is {maximum_level - 1}")
+        super().__init__(f"cannot move past level {level}: the last public level is {maximum_level - 1}")
         self.level = level
         self.maximum_level = maximum_level
 

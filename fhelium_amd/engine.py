@@ -442,16 +442,60 @@ class CkksEngine:
     def add(self, lhs: Ciphertext, rhs: Ciphertext, *, inplace: bool = False) -> Ciphertext:
         """Component-wise canonical sum at equal scale and state (ckks_engine.py:1708-1745, 3352-3377)."""
 
-        self._same_layout(lhs, rhs)
-        if lhs.scale != rhs.scale:
-            raise ScaleMismatchError(f"add: scales differ ({lhs.scale} != {rhs.scale})")
-        if lhs.component_count != rhs.component_count:
-            raise CkksStateError("add: component counts differ")
+        self._same_state(lhs, rhs, "add")
         params = self.runtime.params(lhs.level, lhs.includes_p)
         result = lhs if inplace else lhs.clone()
         flat = result.data.view(-1, result.data.size(-2), result.data.size(-1))
         K.add_canonical_(flat, rhs.data.reshape(flat.shape), params)
         return result
+
+    def _same_state(self, lhs: Ciphertext, rhs: Ciphertext, operation: str) -> None:
+        """Equal layout, scale and arithmetic state (ckks_engine.py
+        _assert_same_cipher_layout)."""
+
+        self._same_layout(lhs, rhs)
+        if lhs.scale != rhs.scale:
+            raise ScaleMismatchError(f"{operation}: scales differ ({lhs.scale} != {rhs.scale})")
+        if lhs.component_count != rhs.component_count:
+            raise CkksStateError(f"{operation}: component counts differ")
+        if (lhs.polynomial_domain, lhs.residue_representation) != (rhs.polynomial_domain,
+                                                                  rhs.residue_representation):
+            raise CkksStateError(f"{operation}: operand states differ ({lhs.polynomial_domain}/"
+                                 f"{lhs.residue_representation} vs {rhs.polynomial_domain}/"
+                                 f"{rhs.residue_representation})")
+
+    def subtract(self, lhs: Ciphertext, rhs: Ciphertext, *, inplace: bool = False) -> Ciphertext:
+        """Component-wise canonical difference at equal scale and state
+        (ckks_engine.py:1871-1917)."""
+
+        self._same_state(lhs, rhs, "subtract")
+        params = self.runtime.params(lhs.level, lhs.includes_p)
+        result = lhs if inplace else lhs.clone()
+        flat = result.data.view(-1, result.data.size(-2), result.data.size(-1))
+        K.sub_canonical_(flat, rhs.data.reshape(flat.shape), params)
+        return result
+
+    def subtract_(self, lhs: Ciphertext, rhs: Ciphertext) -> Ciphertext:
+        """subtract in place into ``lhs`` (ckks_engine.py:1910)."""
+
+        return self.subtract(lhs, rhs, inplace=True)
+
+    def negate(self, ct: Ciphertext, *, inplace: bool = False) -> Ciphertext:
+        """-c_j mod q, canonical, state unchanged (ckks_engine.py:1919-1957)."""
+
+        params = self.runtime.params(ct.level, ct.includes_p)
+        zero = torch.zeros_like(ct.data)
+        flat = zero.view(-1, zero.size(-2), zero.size(-1))
+        K.sub_canonical_(flat, ct.data.reshape(flat.shape), params)
+        if inplace:
+            ct.data.copy_(zero)
+            return ct
+        return replace(ct, data=zero)
+
+    def negate_(self, ct: Ciphertext) -> Ciphertext:
+        """negate in place (ckks_engine.py:1959)."""
+
+        return self.negate(ct, inplace=True)
 
     # ------------------------------------------------------------------
     # Domain transitions (ckks_engine.py:867-985, 2790-2835): strict source
@@ -671,12 +715,52 @@ class CkksEngine:
         return Ciphertext(data, target_level, ct.scale, ct.polynomial_domain, ct.residue_representation,
                           ct.modulus_basis, ct.context_id)
 
+    def mod_switch_to_level_(self, ct: Ciphertext, target_level: int) -> Ciphertext:
+        """mod_switch_to_level in place (ckks_engine.py:1317-1344): ``ct.data``
+        becomes the view of the kept rows of its own allocation; the current
+        level is a no-op."""
+
+        switched = self.mod_switch_to_level(ct, target_level)  # validates
+        ct.data = ct.data[..., target_level - ct.level:, :]
+        ct.level = switched.level
+        return ct
+
+    def _next_level(self, ct: Ciphertext) -> int:
+        if ct.level + 1 >= self.config.public_level_count:
+            raise MaximumLevelError(ct.level, self.config.public_level_count)
+        return ct.level + 1
+
+    def mod_switch_to_next_level(self, ct: Ciphertext) -> Ciphertext:
+        """Drop the leading Q row, keeping the scale; MaximumLevelError at the
+        final level (ckks_engine.py:1208-1246)."""
+
+        return self.mod_switch_to_level(ct, self._next_level(ct))
+
+    def mod_switch_to_next_level_(self, ct: Ciphertext) -> Ciphertext:
+        """mod_switch_to_next_level in place (ckks_engine.py:1248-1276)."""
+
+        return self.mod_switch_to_level_(ct, self._next_level(ct))
+
     def reinterpret_at_scale(self, ct: Ciphertext, target_scale: float, *,
                              max_relative_change: float | None = None) -> Ciphertext:
         """Same residues read at ``target_scale``; with a bound, the symmetric
         relative change max(old/new, new/old) - 1 must not exceed it
         (ckks_engine.py:1346-1393, 3264-3300)."""
 
+        out = ct.clone()
+        out.scale = self._reinterpreted_scale(ct, target_scale, max_relative_change)
+        return out
+
+    def reinterpret_at_scale_(self, ct: Ciphertext, target_scale: float, *,
+                              max_relative_change: float | None = None) -> Ciphertext:
+        """reinterpret_at_scale on ``ct`` itself: only its scale changes
+        (ckks_engine.py:1396-1430)."""
+
+        ct.scale = self._reinterpreted_scale(ct, target_scale, max_relative_change)
+        return ct
+
+    @staticmethod
+    def _reinterpreted_scale(ct: Ciphertext, target_scale: float, max_relative_change: float | None) -> float:
         target = float(target_scale)
         if not (math.isfinite(target) and target > 0):
             raise InvalidScaleError(f"target scale must be positive and finite, got {target_scale}")
@@ -686,9 +770,7 @@ class CkksEngine:
                 raise ValueError("max_relative_change must be a non-negative finite float")
             if max(ct.scale / target, target / ct.scale) - 1.0 > limit:
                 raise ScaleMismatchError(f"reinterpret_at_scale: {ct.scale} -> {target} exceeds {limit}")
-        out = ct.clone()
-        out.scale = target
-        return out
+        return target
 
     def _assert_rotation_ciphertext(self, ct: Ciphertext) -> None:
         ct.assert_state(polynomial_domain="coefficient", modulus_basis="Q", residue_representation="standard",

@@ -1,9 +1,10 @@
-"""The reference engine's in-place and key-switch forms of the hot path
-(fhelium/engine/ckks_engine.py:1131-1246, 1677-1706, 1752, 2068, 2170,
+"""The reference engine's in-place, linear and key-switch forms of the hot
+path (fhelium/engine/ckks_engine.py:1131-1430, 1677-1760, 1871-2175,
 2333-2346): rescale with nearest and floor rounding against the oracle,
 in-place rescale and rotation equal to their functional forms (in-place
 rescale writes into the original allocation), switch_key equal to the key
-switch inside a rotation, and the in-place additions and plaintext product."""
+switch inside a rotation, the in-place additions and plaintext product,
+subtract and negate identities, the mod-switch forms and reinterpret_at_scale_."""
 
 from __future__ import annotations
 
@@ -118,3 +119,94 @@ def test_in_place_additions_and_plaintext_product(eng):
     want = eng.multiply_plaintext(na, pt_ntt)
     work = na.clone()
     assert eng.multiply_plaintext_(work, pt_ntt) is work and torch.equal(work.data, want.data)
+
+
+def test_subtract_and_negate_identities(eng):
+    """subtract and negate (ckks_engine.py:1871-1960): (a - b) + b == a and
+    a + (-a) == 0 bit for bit, -(-a) == a, residues stay canonical, the
+    in-place forms equal the functional ones, and the state is kept."""
+
+    a, b = _ct(eng, 6, seed=11), _ct(eng, 6, seed=12)
+    d = eng.subtract(a, b)
+    assert torch.equal(eng.add(d, b).data, a.data)
+    n = eng.negate(a)
+    assert (n.level, n.scale, n.polynomial_domain) == (a.level, a.scale, a.polynomial_domain)
+    assert torch.equal(eng.negate(n).data, a.data)
+    assert not torch.any(eng.add(a, n).data)
+    q = torch.tensor(H.moduli_array(eng.config, 6, a.data.size(-2)), dtype=torch.int64,
+                     device=a.data.device).view(-1, 1)
+    assert bool(((n.data >= 0) & (n.data < q)).all()) and bool(((d.data >= 0) & (d.data < q)).all())
+    want = (a.data - b.data).remainder(q)
+    assert torch.equal(d.data, want)
+    work = a.clone()
+    assert eng.subtract_(work, b) is work and torch.equal(work.data, d.data)
+    work = a.clone()
+    storage = work.data.data_ptr()
+    assert eng.negate_(work) is work and work.data.data_ptr() == storage and torch.equal(work.data, n.data)
+    na, nb = eng.coefficient_domain_to_ntt_domain(a), eng.coefficient_domain_to_ntt_domain(b)
+    dn = eng.subtract(na, nb)
+    assert (dn.polynomial_domain, dn.residue_representation) == ("ntt", "montgomery")
+    assert torch.equal(eng.ntt_domain_to_coefficient_domain(dn).data, d.data)  # linearity of the NTT
+
+
+def test_subtract_rejects_mismatched_operands(eng):
+    from fhelium_amd.core import ScaleMismatchError
+
+    a, b = _ct(eng, 2, seed=1), _ct(eng, 2, seed=2)
+    with pytest.raises(CkksStateError):
+        eng.subtract(a, eng.coefficient_domain_to_ntt_domain(b))
+    with pytest.raises(CkksStateError):
+        eng.add(a, eng.coefficient_domain_to_ntt_domain(b))
+    other = b.clone()
+    other.scale = b.scale * 2
+    with pytest.raises(ScaleMismatchError):
+        eng.subtract(a, other)
+    with pytest.raises(ValueError, match="layouts differ"):
+        eng.subtract(a, _ct(eng, 3, seed=2))
+
+
+def test_mod_switch_forms(eng):
+    """mod_switch_to_next_level / _to_level_ (ckks_engine.py:1208-1344):
+    the kept rows are the input's, the in-place form narrows a view of its
+    own allocation, and the final level raises MaximumLevelError."""
+
+    from fhelium_amd.core import MaximumLevelError
+
+    ct = _ct(eng, 5, seed=21)
+    nxt = eng.mod_switch_to_next_level(ct)
+    assert nxt.level == 6 and nxt.scale == ct.scale and torch.equal(nxt.data, ct.data[..., 1:, :])
+    assert nxt.data.data_ptr() != ct.data.data_ptr()
+    work = ct.clone()
+    base = work.data.data_ptr()
+    assert eng.mod_switch_to_next_level_(work) is work
+    assert work.level == 6 and work.data.data_ptr() == base + work.data.size(-1) * 8
+    assert eng.mod_switch_to_level_(work, 9) is work and work.level == 9
+    assert torch.equal(work.data, ct.data[..., 4:, :])
+    assert eng.mod_switch_to_level_(work, 9) is work and work.level == 9  # current level: no-op
+    last = eng.config.public_level_count - 1
+    top = eng.mod_switch_to_level(ct, last)
+    with pytest.raises(MaximumLevelError):
+        eng.mod_switch_to_next_level(top)
+    with pytest.raises(MaximumLevelError):
+        eng.mod_switch_to_next_level_(top)
+    with pytest.raises(ValueError):
+        eng.mod_switch_to_level_(work, 3)  # back up the chain
+
+
+def test_reinterpret_at_scale_in_place(eng):
+    """reinterpret_at_scale_ (ckks_engine.py:1396-1430) changes the scale
+    only, with the functional form's guard."""
+
+    from fhelium_amd.core import InvalidScaleError, ScaleMismatchError
+
+    ct = _ct(eng, 1, seed=3)
+    before = ct.data.clone()
+    target = ct.scale * (1 + 1e-13)
+    want = eng.reinterpret_at_scale(ct, target, max_relative_change=1e-12)
+    assert eng.reinterpret_at_scale_(ct, target, max_relative_change=1e-12) is ct
+    assert ct.scale == want.scale == target and torch.equal(ct.data, before)
+    with pytest.raises(ScaleMismatchError):
+        eng.reinterpret_at_scale_(ct, target * 2, max_relative_change=1e-12)
+    with pytest.raises(InvalidScaleError):
+        eng.reinterpret_at_scale_(ct, -1.0)
+    assert ct.scale == target  # rejected calls leave the scale as it was
