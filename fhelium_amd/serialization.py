@@ -19,9 +19,11 @@ ciphertexts written by either backend load on the other:
 
 ``context_id`` is the SHA-256 context fingerprint (``CkksConfig.context_id``).
 Values here do not carry their context, so ``save_value`` takes it as
-``context`` (a ``CkksConfig`` or an engine).  Files are written atomically
-(temporary file, then rename) with mode 0600; secret keys need
-``allow_secret=True``.
+``context`` (a ``CkksConfig`` or an engine).  A file is staged under a
+hidden name in the destination directory, owner-only (0600) from its
+creation, and published in one step: a hard link when an existing file must
+not be replaced (so a file that appears meanwhile is never clobbered), a
+rename otherwise.  Secret keys need ``allow_secret=True``.
 """
 
 from __future__ import annotations
@@ -29,7 +31,7 @@ from __future__ import annotations
 import json
 import math
 import os
-import tempfile
+import secrets
 from dataclasses import dataclass
 from pathlib import Path
 from typing import Any
@@ -112,18 +114,9 @@ def save_value(value, path: str | os.PathLike[str], *, context, allow_secret: bo
     cfg = _config_of(context)
     value_type, value_meta, data = _describe(value, cfg)
     if value_type == "SecretKey" and not allow_secret:
-        raise PermissionError("SecretKey persistence is disabled by default; pass allow_secret=True "
-                              "only for a path with an at-rest security policy")
-    dest = Path(path).expanduser()
-    if not dest.parent.exists():
-        raise FileNotFoundError(f"Value file parent directory does not exist: {dest.parent}")
-    if not dest.parent.is_dir():
-        raise NotADirectoryError(dest.parent)
-    if dest.exists():
-        if dest.is_dir():
-            raise IsADirectoryError(dest)
-        if not overwrite:
-            raise FileExistsError(dest)
+        raise PermissionError("writing a SecretKey is disabled by default: the file holds the key "
+                              "unencrypted, so pass allow_secret=True only for protected storage")
+    dest = _writable_destination(Path(path).expanduser(), overwrite)
     host = data.detach().to("cpu").contiguous()
     if str(host.dtype) not in _DTYPES:
         raise TypeError(f"value payloads are int64 or int32 residues, got {host.dtype}")
@@ -138,21 +131,53 @@ def save_value(value, path: str | os.PathLike[str], *, context, allow_secret: bo
         "value_metadata": value_meta,
     }
     meta = _parse_manifest(manifest)
-    fd, tmp_name = tempfile.mkstemp(prefix=f".{dest.name}.", suffix=".tmp", dir=dest.parent)
-    os.close(fd)
-    tmp = Path(tmp_name)
-    try:
-        save_file({"data": host}, str(tmp), metadata={
-            _FORMAT_KEY: FILE_FORMAT, _SCHEMA_KEY: str(FILE_SCHEMA_VERSION),
-            _MANIFEST_KEY: json.dumps(manifest, sort_keys=True, separators=(",", ":"))})
-        os.chmod(tmp, 0o600)
-        if dest.exists() and not overwrite:
-            raise FileExistsError(dest)
-        os.replace(tmp, dest)
-    except BaseException:
-        tmp.unlink(missing_ok=True)
-        raise
+    header = {_FORMAT_KEY: FILE_FORMAT, _SCHEMA_KEY: str(FILE_SCHEMA_VERSION),
+              _MANIFEST_KEY: json.dumps(manifest, sort_keys=True, separators=(",", ":"))}
+    _publish(dest, overwrite, lambda staged: save_file({"data": host}, str(staged), metadata=header))
     return meta
+
+
+def _writable_destination(dest: Path, overwrite: bool) -> Path:
+    """Reject a destination that cannot take a value file before any data is
+    moved to the host (the error types of safetensors.py:67-79)."""
+
+    if not dest.parent.exists():
+        raise FileNotFoundError(f"cannot write {dest.name}: parent directory {dest.parent} is missing")
+    if not dest.parent.is_dir():
+        raise NotADirectoryError(f"cannot write {dest.name}: {dest.parent} is not a directory")
+    if dest.is_dir():
+        raise IsADirectoryError(f"{dest} is a directory, not a value file")
+    if dest.exists() and not overwrite:
+        raise FileExistsError(f"{dest} exists; pass overwrite=True to replace it")
+    return dest
+
+
+def _publish(dest: Path, overwrite: bool, write) -> None:
+    """Run ``write(staged)`` on a fresh owner-only file beside ``dest``, then
+    make it ``dest`` in one step; the staged name never outlives the call."""
+
+    staged = dest.with_name(f".{dest.name}.{secrets.token_hex(6)}.partial")
+    fd = os.open(staged, os.O_WRONLY | os.O_CREAT | os.O_EXCL, 0o600)
+    try:
+        os.fchmod(fd, 0o600)  # independent of the process umask
+    finally:
+        os.close(fd)
+    try:
+        write(staged)
+        if overwrite:
+            os.replace(staged, dest)
+        else:
+            try:
+                os.link(staged, dest)  # fails if dest appeared since the check
+            except FileExistsError:
+                raise FileExistsError(f"{dest} appeared while it was being written") from None
+            except OSError:
+                # a file system without hard links: fall back to a checked rename
+                if dest.exists():
+                    raise FileExistsError(f"{dest} appeared while it was being written") from None
+                os.replace(staged, dest)
+    finally:
+        staged.unlink(missing_ok=True)
 
 
 def _parse_manifest(manifest: Any) -> ValueFileMetadata:

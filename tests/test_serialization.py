@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import json
 import re
+import stat
 
 import pytest
 import torch
@@ -99,6 +100,29 @@ def test_file_lifecycle_policy(tmp_path):
     replacement = Ciphertext(torch.ones_like(value.data), value.level, value.scale)
     save_value(replacement, path, context=CFG, overwrite=True)
     assert torch.equal(load_value(path, context=CFG).data, replacement.data)
+    assert stat.S_IMODE(path.stat().st_mode) == 0o600
+    assert [p.name for p in path.parent.iterdir()] == ["ct.safetensors"]  # no staged file left
+    with pytest.raises(IsADirectoryError):
+        save_value(value, path.parent, context=CFG, overwrite=True)
+
+
+def test_failed_write_leaves_destination_alone(tmp_path, monkeypatch):
+    """A write that fails part-way removes its staged file and leaves an
+    existing destination as it was."""
+
+    from fhelium_amd import serialization as S
+
+    path = tmp_path / "ct.safetensors"
+    save_value(_ct(), path, context=CFG)
+    before = path.read_bytes()
+
+    def broken(*args, **kwargs):
+        raise OSError("disk full")
+
+    monkeypatch.setattr(S, "save_file", broken)
+    with pytest.raises(OSError, match="disk full"):
+        save_value(Ciphertext(torch.ones_like(_ct().data), 3, 2.0 ** 40), path, context=CFG, overwrite=True)
+    assert path.read_bytes() == before and [p.name for p in tmp_path.iterdir()] == ["ct.safetensors"]
 
 
 def test_secret_key_needs_opt_in(tmp_path):
