@@ -223,6 +223,12 @@ class RelinearizationKey(KeySwitchKey):
 
 
 @dataclass
+class ConjugationKey(KeySwitchKey):
+    """Key switch material from sigma_{-1}(s) back to s (keys.py:386-391):
+    after X -> X^{-1} it restores the key relation, conjugating every slot."""
+
+
+@dataclass
 class RotationKey(KeySwitchKey):
     rotation_step: int = 0
     galois_element: int = 1

@@ -24,7 +24,7 @@ from fhelium_amd import kernels as K
 from fhelium_amd import rns
 from fhelium_amd.config import CkksConfig, Preset
 from fhelium_amd.csprng import Csprng
-from fhelium_amd.core import (Ciphertext, CkksStateError, InvalidScaleError, KeySwitchKey,
+from fhelium_amd.core import (Ciphertext, CkksStateError, ConjugationKey, InvalidScaleError, KeySwitchKey,
                               MaximumLevelError, Plaintext, RelinearizationKey, RotationKey,
                               ScaleMismatchError, SecretKey)
 from fhelium_amd.keyswitch import HybridKeySwitcher
@@ -60,6 +60,7 @@ class CkksEngine:
         self._moduli_cache: dict[tuple[int, ...], torch.Tensor] = {}
         self.secret_key: SecretKey | None = None
         self.relinearization_key: RelinearizationKey | None = None
+        self.conjugation_key: ConjugationKey | None = None
         self.rotation_keys: dict[int, RotationKey] = {}
         self.galois_generator = 3
 
@@ -168,17 +169,44 @@ class CkksEngine:
         self.relinearization_key = RelinearizationKey(self.create_key_switch_key(s2))
         return self.relinearization_key
 
+    def _galois_key_data(self, galois_element: int, sk: SecretKey) -> torch.Tensor:
+        """Key switch material from sigma_g(s) back to s (key_generator.py:243-326)."""
+
+        idx, sgn = self.runtime.coefficient_galois(galois_element)
+        s_rot = sk.coefficients[idx.long()] * sgn.to(torch.int64)
+        src = self._to_ntt_rows(self._lift_signed(s_rot, tuple(range(self.config.total_num_primes))), 0)
+        return self.create_key_switch_key(src)
+
     def generate_rotation_key(self, rotation_step: int) -> RotationKey:
         cfg = self.config
         sk = self._require_secret()
         step = RotationKey.canonical_step(rotation_step, cfg.N)
         g = rns.rotation_galois_element(cfg.N, step, self.galois_generator)
-        idx, sgn = self.runtime.coefficient_galois(g)
-        s_rot = sk.coefficients[idx.long()] * sgn.to(torch.int64)
-        src = self._to_ntt_rows(self._lift_signed(s_rot, tuple(range(cfg.total_num_primes))), 0)
-        key = RotationKey(self.create_key_switch_key(src), rotation_step=step, galois_element=g)
+        key = RotationKey(self._galois_key_data(g, sk), rotation_step=step, galois_element=g)
         self.rotation_keys[step] = key
         return key
+
+    @property
+    def conjugation_galois_element(self) -> int:
+        """2N - 1: the automorphism X -> X^{-1}, which conjugates every slot."""
+
+        return 2 * self.config.N - 1
+
+    def create_conjugation_key(self, secret_key: SecretKey | None = None) -> ConjugationKey:
+        """Key switch material from sigma_{-1}(s) back to s, not installed
+        (ckks_engine.py:362-376, core/keys.py:386-391); the engine's secret
+        key by default."""
+
+        sk = self._require_secret() if secret_key is None else secret_key
+        if not isinstance(sk, SecretKey):
+            raise TypeError(f"expected SecretKey, got {type(sk).__name__}")
+        return ConjugationKey(self._galois_key_data(self.conjugation_galois_element, sk))
+
+    def generate_conjugation_key(self) -> ConjugationKey:
+        """create_conjugation_key with the engine's secret key, installed."""
+
+        self.conjugation_key = self.create_conjugation_key()
+        return self.conjugation_key
 
     def rotation_key(self, rotation_step: int) -> RotationKey:
         step = RotationKey.canonical_step(rotation_step, self.config.N)
@@ -446,8 +474,19 @@ class CkksEngine:
         params = self.runtime.params(lhs.level, lhs.includes_p)
         result = lhs if inplace else lhs.clone()
         flat = result.data.view(-1, result.data.size(-2), result.data.size(-1))
-        K.add_canonical_(flat, rhs.data.reshape(flat.shape), params)
+        K.add_canonical_(flat, self._operand_for(result, rhs), params)
         return result
+
+    @staticmethod
+    def _operand_for(result: Ciphertext, rhs: Ciphertext) -> torch.Tensor:
+        """``rhs``'s payload in ``result``'s flat shape; a copy when it shares
+        storage with ``result`` (``add_(a, a)``), since the in-place kernels
+        refuse an output that overlaps a read-only operand."""
+
+        other = rhs.data
+        if other.untyped_storage().data_ptr() == result.data.untyped_storage().data_ptr():
+            other = other.clone()
+        return other.reshape(-1, result.data.size(-2), result.data.size(-1))
 
     def _same_state(self, lhs: Ciphertext, rhs: Ciphertext, operation: str) -> None:
         """Equal layout, scale and arithmetic state (ckks_engine.py
@@ -472,7 +511,7 @@ class CkksEngine:
         params = self.runtime.params(lhs.level, lhs.includes_p)
         result = lhs if inplace else lhs.clone()
         flat = result.data.view(-1, result.data.size(-2), result.data.size(-1))
-        K.sub_canonical_(flat, rhs.data.reshape(flat.shape), params)
+        K.sub_canonical_(flat, self._operand_for(result, rhs), params)
         return result
 
     def subtract_(self, lhs: Ciphertext, rhs: Ciphertext) -> Ciphertext:
@@ -837,6 +876,24 @@ class CkksEngine:
         twice = rt.twice_modulus[rt.row_start(ct.level): self.config.num_q_primes]
         rotated = K.apply_coefficient_galois_automorphism(ct.data, idx, sgn, twice)
         return self._switch_rotated(ct, rotated, key)
+
+    def conjugate(self, ct: Ciphertext, key: ConjugationKey | None = None) -> Ciphertext:
+        """m'_j = conj(m_j): X -> X^{-1} on both components, then the key
+        switch from sigma_{-1}(s) back to s (ckks_engine.py:2348-2400); the
+        installed conjugation key by default."""
+
+        self._assert_rotation_ciphertext(ct)
+        key = self.conjugation_key if key is None else key
+        if not isinstance(key, ConjugationKey):
+            raise TypeError("conjugate needs a ConjugationKey (generate_conjugation_key installs one)")
+        g = self.conjugation_galois_element
+        if self.fused_keyswitch:
+            entries = int(np.prod(ct.batch_shape)) if ct.batch_shape else 1
+            return self._switch_rotated_fused(ct, key, [g] * entries)
+        rt = self.runtime
+        idx, sgn = rt.coefficient_galois(g)
+        twice = rt.twice_modulus[rt.row_start(ct.level): self.config.num_q_primes]
+        return self._switch_rotated(ct, K.apply_coefficient_galois_automorphism(ct.data, idx, sgn, twice), key)
 
     def _switch_rotated_fused(self, ct: Ciphertext, key, galois_elements: list[int]) -> Ciphertext:
         """Automorphism folded into the key switch: the kernels read c1 and

@@ -41,7 +41,7 @@ from safetensors import safe_open
 from safetensors.torch import save_file
 
 from fhelium_amd import rns
-from fhelium_amd.core import (Ciphertext, KeySwitchKey, Plaintext, RelinearizationKey, RotationKey,
+from fhelium_amd.core import (Ciphertext, ConjugationKey, KeySwitchKey, Plaintext, RelinearizationKey, RotationKey,
                               SecretKey)
 
 FILE_FORMAT = "fhelium-value"
@@ -50,7 +50,7 @@ VALUE_SCHEMA_VERSION = 2
 _FORMAT_KEY = "fhelium.format"
 _SCHEMA_KEY = "fhelium.schema_version"
 _MANIFEST_KEY = "fhelium.manifest"
-_KEY_TYPES = ("SecretKey", "KeySwitchKey", "RotationKey", "RelinearizationKey")
+_KEY_TYPES = ("SecretKey", "KeySwitchKey", "RotationKey", "RelinearizationKey", "ConjugationKey")
 _VALUE_TYPES = ("Plaintext", "Ciphertext") + _KEY_TYPES
 _DTYPES = {"torch.int64": "I64", "torch.int32": "I32"}
 
@@ -102,6 +102,8 @@ def _describe(value, cfg) -> tuple[str, dict[str, Any], torch.Tensor]:
         return "RotationKey", dict(key_meta, rotation_step=int(value.rotation_step)), value.data
     if isinstance(value, RelinearizationKey):
         return "RelinearizationKey", key_meta, value.data
+    if isinstance(value, ConjugationKey):
+        return "ConjugationKey", key_meta, value.data
     if isinstance(value, KeySwitchKey):
         return "KeySwitchKey", key_meta, value.data
     raise TypeError(f"value files hold {list(_VALUE_TYPES)}; got {type(value).__name__}")
@@ -327,4 +329,6 @@ def load_value(path: str | os.PathLike[str], *, context, device: torch.device | 
         return RotationKey(data, rotation_step=step, galois_element=rns.rotation_galois_element(cfg.N, step, 3))
     if vt == "RelinearizationKey":
         return RelinearizationKey(data)
+    if vt == "ConjugationKey":
+        return ConjugationKey(data)
     return KeySwitchKey(data)

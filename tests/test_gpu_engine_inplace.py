@@ -210,3 +210,15 @@ def test_reinterpret_at_scale_in_place(eng):
     with pytest.raises(InvalidScaleError):
         eng.reinterpret_at_scale_(ct, -1.0)
     assert ct.scale == target  # rejected calls leave the scale as it was
+
+
+def test_in_place_with_itself(eng):
+    """add_(a, a) doubles and subtract_(a, a) clears, as the reference's
+    elementwise in-place forms do."""
+
+    a = _ct(eng, 7, seed=30)
+    twice = eng.add(a, a)
+    work = a.clone()
+    assert torch.equal(eng.add_(work, work).data, twice.data)
+    work = a.clone()
+    assert not torch.any(eng.subtract_(work, work).data)

@@ -14,8 +14,8 @@ import torch
 from safetensors import safe_open
 from safetensors.torch import load_file, save_file
 
-from fhelium_amd.core import (Ciphertext, KeySwitchKey, Plaintext, RelinearizationKey, RotationKey,
-                              SecretKey)
+from fhelium_amd.core import (Ciphertext, ConjugationKey, KeySwitchKey, Plaintext, RelinearizationKey,
+                              RotationKey, SecretKey)
 from fhelium_amd.serialization import inspect_value, load_value, save_value
 from tests import helpers as H
 
@@ -47,6 +47,7 @@ VALUES = {
     "rotation-key": lambda: _key(RotationKey, rotation_step=-5, galois_element=0),
     "relinearization-key": lambda: _key(RelinearizationKey),
     "key-switch-key": lambda: _key(KeySwitchKey),
+    "conjugation-key": lambda: _key(ConjugationKey),
 }
 
 
