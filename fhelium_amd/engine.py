@@ -14,6 +14,7 @@ come from an external package that is not available here).
 from __future__ import annotations
 
 import math
+import numbers
 from dataclasses import replace
 
 import numpy as np
@@ -514,6 +515,51 @@ class CkksEngine:
         K.sub_canonical_(flat, self._operand_for(result, rhs), params)
         return result
 
+    def sum_ciphertexts(self, ciphertexts) -> Ciphertext:
+        """Canonical sum of equal-state ciphertexts into new storage
+        (ckks_engine.py:1761-1797)."""
+
+        if not ciphertexts:
+            raise ValueError("sum_ciphertexts needs at least one ciphertext (an empty sum has no level or scale)")
+        if not all(isinstance(ct, Ciphertext) for ct in ciphertexts):
+            raise TypeError("sum_ciphertexts sums Ciphertext values")
+        result = ciphertexts[0].clone()
+        for ct in ciphertexts[1:]:
+            self.add(result, ct, inplace=True)
+        return result
+
+    def sum_ciphertext_batch(self, batch: Ciphertext, *, dim: int = 0) -> Ciphertext:
+        """Sum over one batch axis (ckks_engine.py:1801-1869).  The axis is
+        moved to the front of one working copy, then halves are added in
+        place, so each round is one canonical-add launch over all remaining
+        entries; an odd count's last entry is added at the end.  Modular
+        addition is exact, so the residues equal any other summation order."""
+
+        if not isinstance(batch, Ciphertext):
+            raise TypeError(f"sum_ciphertext_batch expects a Ciphertext, got {type(batch).__name__}")
+        axes = len(batch.batch_shape)
+        if axes == 0:
+            raise ValueError("sum_ciphertext_batch requires a batched value")
+        d = dim + axes if dim < 0 else dim
+        if not 0 <= d < axes:
+            raise IndexError(f"batch dimension {dim} is outside shape {batch.batch_shape}")
+        params = self.runtime.params(batch.level, batch.includes_p)
+        limbs, n = batch.data.size(-2), batch.data.size(-1)
+        work = batch.data.movedim(d + 1, 0).contiguous()
+        tails = []
+        while work.size(0) > 1:
+            count = work.size(0)
+            half = count // 2
+            if count % 2:
+                tails.append(work[count - 1])  # beyond every later round's rows
+            K.add_canonical_(work[:half].view(-1, limbs, n), work[half: 2 * half].reshape(-1, limbs, n), params)
+            work = work[:half]
+        total = work[0].clone()
+        flat = total.view(-1, limbs, n)
+        for tail in tails:
+            K.add_canonical_(flat, tail.reshape(-1, limbs, n), params)
+        return replace(batch, data=total)
+
     def subtract_(self, lhs: Ciphertext, rhs: Ciphertext) -> Ciphertext:
         """subtract in place into ``lhs`` (ckks_engine.py:1910)."""
 
@@ -567,6 +613,11 @@ class CkksEngine:
                 raise CkksStateError(f"{operation} requires Montgomery ciphertext residues")
         else:
             raise TypeError(f"Domain transition expects Ciphertext or Plaintext, got {type(value).__name__}")
+        self._check_rows(value, operation)
+
+    def _check_rows(self, value, operation: str) -> None:
+        """The payload holds exactly the rows of its level and basis."""
+
         self.layout.check_level(value.level)
         rows = value.data.size(-2)
         start = self.runtime.row_start(value.level)
@@ -636,6 +687,55 @@ class CkksEngine:
 
         self._domain_source(value, "ntt")
         return self._inverse_ntt_(value)
+
+    def _residue_source(self, plaintext, source: str) -> None:
+        """A coefficient-domain plaintext in ``source`` residues
+        (ckks_engine.py:2835-2857)."""
+
+        operation = ("standard_residues_to_montgomery_residues" if source == "standard"
+                     else "montgomery_residues_to_standard_residues")
+        if not isinstance(plaintext, Plaintext):
+            raise TypeError(f"{operation} expects a Plaintext, got {type(plaintext).__name__}")
+        if plaintext.modulus_basis not in ("Q", "QP"):
+            raise ValueError(f"Unsupported plaintext modulus_basis: {plaintext.modulus_basis!r}")
+        if plaintext.polynomial_domain != "coefficient":
+            raise ValueError(f"{operation} requires a coefficient-domain plaintext")
+        if plaintext.residue_representation != source:
+            raise ValueError(f"{operation} requires {'standard' if source == 'standard' else 'Montgomery'} residues")
+        self._check_rows(plaintext, operation)
+
+    def _convert_residues_(self, plaintext: Plaintext, target: str) -> Plaintext:
+        params = self.runtime.params(plaintext.level, plaintext.modulus_basis == "QP")
+        rows = plaintext.data.view(-1, plaintext.data.size(-2), plaintext.data.size(-1))
+        (K.to_montgomery_ if target == "montgomery" else K.from_montgomery_)(rows, params)
+        plaintext.residue_representation = target
+        return plaintext
+
+    def standard_residues_to_montgomery_residues(self, plaintext: Plaintext) -> Plaintext:
+        """r -> r R mod q on every row of a coefficient-domain plaintext, into
+        new storage (ckks_engine.py:987-1003, 3134-3150)."""
+
+        self._residue_source(plaintext, "standard")
+        return self._convert_residues_(replace(plaintext, data=plaintext.data.clone()), "montgomery")
+
+    def standard_residues_to_montgomery_residues_(self, plaintext: Plaintext) -> Plaintext:
+        """standard_residues_to_montgomery_residues in place (ckks_engine.py:1005-1019)."""
+
+        self._residue_source(plaintext, "standard")
+        return self._convert_residues_(plaintext, "montgomery")
+
+    def montgomery_residues_to_standard_residues(self, plaintext: Plaintext) -> Plaintext:
+        """Montgomery reduction of every residue to its canonical value, into
+        new storage (ckks_engine.py:1021-1039, 3170-3186)."""
+
+        self._residue_source(plaintext, "montgomery")
+        return self._convert_residues_(replace(plaintext, data=plaintext.data.clone()), "standard")
+
+    def montgomery_residues_to_standard_residues_(self, plaintext: Plaintext) -> Plaintext:
+        """montgomery_residues_to_standard_residues in place (ckks_engine.py:1041-1054)."""
+
+        self._residue_source(plaintext, "montgomery")
+        return self._convert_residues_(plaintext, "standard")
 
     # ------------------------------------------------------------------
     # The hot path
@@ -763,6 +863,42 @@ class CkksEngine:
         ct.data = ct.data[..., target_level - ct.level:, :]
         ct.level = switched.level
         return ct
+
+    @property
+    def public_level_count(self) -> int:
+        return self.config.public_level_count
+
+    @property
+    def final_public_level(self) -> int:
+        return self.config.public_level_count - 1
+
+    @property
+    def num_slots(self) -> int:
+        return self.config.slots
+
+    def rescale_to_next_drop_prime(self, *, level: int) -> int:
+        """The Q prime a rescale at ``level`` divides by (ckks_engine.py:1056-1087)."""
+
+        if type(level) is not int:
+            raise TypeError("level must be an integer")
+        if level < 0:
+            raise ValueError(f"level must be non-negative, got {level}")
+        if level >= self.final_public_level:
+            raise MaximumLevelError(level, self.config.public_level_count)
+        return int(self.config.moduli[self.runtime.row_start(level)])
+
+    def rescale_to_next_output_scale(self, input_scale: float, *, level: int) -> float:
+        """input_scale / q_drop in binary64, as rescale_to_next_level sets it
+        (ckks_engine.py:1089-1129)."""
+
+        real = isinstance(input_scale, numbers.Real) and not isinstance(input_scale, bool)
+        scale = float(input_scale) if real else math.nan
+        if not (math.isfinite(scale) and scale > 0):
+            raise InvalidScaleError(f"rescale input scale must be positive and finite, got {input_scale!r}")
+        out = scale / float(self.rescale_to_next_drop_prime(level=level))
+        if not (math.isfinite(out) and out > 0):
+            raise InvalidScaleError(f"rescale result scale {out!r} is not positive and finite")
+        return out
 
     def _next_level(self, ct: Ciphertext) -> int:
         if ct.level + 1 >= self.config.public_level_count:
