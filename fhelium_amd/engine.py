@@ -688,6 +688,57 @@ class CkksEngine:
         self._domain_source(value, "ntt")
         return self._inverse_ntt_(value)
 
+    def _on_device(self, t: torch.Tensor) -> bool:
+        """``t`` lives on the engine's device (an index-less device means the
+        current one of its type)."""
+
+        if t.device.type != self.device.type:
+            return False
+        index = self.device.index
+        if index is None and self.device.type == "cuda":
+            index = torch.cuda.current_device()
+        return index is None or t.device.index == index
+
+    def validate_ciphertext(self, ciphertext: Ciphertext) -> None:
+        """Re-check a ciphertext's construction invariants and that it is
+        complete for this engine (device, dtype, level, basis rows, N);
+        no kernel runs and nothing is mutated (ckks_engine.py:2560-2570)."""
+
+        if not isinstance(ciphertext, Ciphertext):
+            raise TypeError(f"Expected Ciphertext, got {type(ciphertext).__name__}")
+        ct = ciphertext
+        Ciphertext(ct.data, ct.level, ct.scale, ct.polynomial_domain, ct.residue_representation,
+                   ct.modulus_basis, ct.context_id)
+        if ct.data.dtype != torch.int64 or not self._on_device(ct.data):
+            raise ValueError(f"ciphertext payload must be int64 on {self.device}, got {ct.data.dtype} "
+                             f"on {ct.data.device}")
+        self._check_rows(ct, "validate_ciphertext")
+
+    def validate_key_switch_key(self, key: KeySwitchKey) -> None:
+        """Key switch material complete for this engine: [digits at level 0,
+        2, QP rows, N] int64 on the engine's device; a rotation key's step
+        canonical and its Galois element the step's (ckks_engine.py:2593-2630)."""
+
+        if not isinstance(key, KeySwitchKey):
+            raise TypeError(f"Expected KeySwitchKey, got {type(key).__name__}")
+        cfg = self.config
+        want = (len(self.layout.digit_specs(0)), 2, cfg.total_num_primes, cfg.N)
+        data = key.data
+        if not isinstance(data, torch.Tensor) or data.dtype != torch.int64 or not self._on_device(data):
+            raise ValueError(f"key material must be an int64 tensor on {self.device}")
+        if tuple(data.shape) != want:
+            raise ValueError(f"key material must be {list(want)}, got {list(data.shape)}")
+        if isinstance(key, RotationKey):
+            if type(key.rotation_step) is not int:
+                raise TypeError(f"RotationKey rotation_step must be an integer, got "
+                                f"{type(key.rotation_step).__name__}")
+            step = RotationKey.canonical_step(key.rotation_step, cfg.N)
+            if key.rotation_step != step:
+                raise ValueError(f"RotationKey rotation_step is not canonical: {key.rotation_step} != {step}")
+            g = rns.rotation_galois_element(cfg.N, step, self.galois_generator)
+            if key.galois_element != g:
+                raise ValueError(f"RotationKey for step {step} carries Galois element {key.galois_element} != {g}")
+
     def _residue_source(self, plaintext, source: str) -> None:
         """A coefficient-domain plaintext in ``source`` residues
         (ckks_engine.py:2835-2857)."""

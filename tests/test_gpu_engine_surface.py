@@ -3,7 +3,8 @@
 transitions of coefficient-domain plaintexts checked against r R = m (mod q)
 with R = 2^62 on sampled residues, batch sums equal to entry-by-entry sums
 bit for bit (odd counts included, along either batch axis), and the
-rescale scale helpers equal to what rescale_to_next_level sets."""
+rescale scale helpers equal to what rescale_to_next_level sets, and the
+ciphertext and key validators."""
 
 from __future__ import annotations
 
@@ -97,3 +98,34 @@ def test_scale_helpers(eng):
         eng.rescale_to_next_drop_prime(level=1.0)
     with pytest.raises(ValueError):
         eng.rescale_to_next_output_scale("2", level=0)
+
+
+def test_validators(eng):
+    """validate_ciphertext / validate_key_switch_key (ckks_engine.py:2560-2630)."""
+
+    from fhelium_amd.core import KeySwitchKey, RotationKey
+
+    ct = eng.encrypt(np.zeros(eng.config.slots), level=3)
+    eng.validate_ciphertext(ct)
+    bad = ct.clone()
+    bad.level = 2  # one limb short for level 2
+    with pytest.raises(ValueError, match="limbs"):
+        eng.validate_ciphertext(bad)
+    bad = ct.clone()
+    bad.scale = -1.0
+    with pytest.raises(ValueError):
+        eng.validate_ciphertext(bad)
+    with pytest.raises(ValueError):
+        eng.validate_ciphertext(Ciphertext(ct.data.cpu(), ct.level, ct.scale))
+    with pytest.raises(TypeError):
+        eng.validate_ciphertext(ct.data)
+    key = eng.rotation_key(5)
+    eng.validate_key_switch_key(key)
+    with pytest.raises(ValueError, match="Galois"):
+        eng.validate_key_switch_key(RotationKey(key.data, rotation_step=5, galois_element=3))
+    with pytest.raises(ValueError, match="canonical"):
+        eng.validate_key_switch_key(RotationKey(key.data, rotation_step=5 + eng.config.slots, galois_element=3))
+    with pytest.raises(ValueError, match="key material"):
+        eng.validate_key_switch_key(KeySwitchKey(key.data[1:]))
+    with pytest.raises(TypeError):
+        eng.validate_key_switch_key(ct)
