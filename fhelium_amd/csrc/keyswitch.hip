@@ -653,6 +653,13 @@ static_assert(64 % kMacThreads == 0, "a tile's threads must lie in one wave");
 __device__ __forceinline__ int mac_e1(int t, int k) { return t + 32 * k; }
 __device__ __forceinline__ int mac_e2(int t, int k) { return ((t >> 2) << 5) | (k << 2) | (t & 3); }
 __device__ __forceinline__ int mac_e3(int t, int k) { return 8 * t + k; }
+// Word of tile element x in the tile's LDS exchange buffer: the low five
+// bits XORed with bits 3..7, a bijection on [0, 256).  Under it each of the
+// three ownership patterns above has the 16 lanes that share an LDS cycle
+// (8-byte words, two tiles per wave at kTilePitch apart) on 16 distinct
+// bank pairs, so no exchange is bank-conflicted; one pad word per 16
+// elements left mac_e2 two-way conflicted.
+__device__ __forceinline__ int mac_lds(int x) { return x ^ ((x >> 3) & 31); }
 // Slot of the twiddle of the pair whose lower element is e at tile stage st:
 // stage st uses 2^st twiddles, stored after those of the earlier stages.
 __device__ __forceinline__ int mac_tw_slot(int st, int e) { return (1 << st) - 1 + (e >> (8 - st)); }
@@ -739,12 +746,12 @@ __device__ __forceinline__ void mac_finish(const KsArgs& a, int64_t j, int64_t b
       for (int k = 0; k < 8; ++k) {
         int64_t bits;
         __builtin_memcpy(&bits, &acc[k], sizeof(bits));
-        sh[tile_pad(dst.slot(k))] = bits;
+        sh[mac_lds(dst.slot(k))] = bits;
       }
       tile_sync();
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const int64_t bits = sh[tile_pad(mac_e3(t, k))];
+        const int64_t bits = sh[mac_lds(mac_e3(t, k))];
         __builtin_memcpy(&acc[k], &bits, sizeof(bits));
       }
     };
@@ -779,10 +786,10 @@ __device__ __forceinline__ void mac_finish(const KsArgs& a, int64_t j, int64_t b
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e3(t, k))] = pi.mid(v[k]);
+    for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e3(t, k))] = pi.mid(v[k]);
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = pi.unmid(sh[tile_pad(mac_e2(t, k))]);
+    for (int k = 0; k < 8; ++k) v[k] = pi.unmid(sh[mac_lds(mac_e2(t, k))]);
     // Round 2 (slots = bits 4..2): stages 3, 4.
 #pragma unroll
     for (int st = 3; st < 5; ++st) {
@@ -795,10 +802,10 @@ __device__ __forceinline__ void mac_finish(const KsArgs& a, int64_t j, int64_t b
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = pi.mid(v[k]);
+    for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e2(t, k))] = pi.mid(v[k]);
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = pi.unmid(sh[tile_pad(mac_e1(t, k))]);
+    for (int k = 0; k < 8; ++k) v[k] = pi.unmid(sh[mac_lds(mac_e1(t, k))]);
     // Round 3 (slots = bits 7..5): stages 5..7.
 #pragma unroll
     for (int st = 5; st < 8; ++st) {
@@ -915,11 +922,11 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const uint64_t h = (hi2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
-        sh[tile_pad(mac_e3(t, k))] = __double_as_longlong(u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32)));
+        sh[mac_lds(mac_e3(t, k))] = __double_as_longlong(u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32)));
       }
       tile_sync();
 #pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
+      for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e1(t, k))]);
     } else {
 #pragma unroll
       for (int k = 0; k < 8; ++k) v[k] = p.unmid(nv[k]);
@@ -950,10 +957,10 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     // The previous digit's last reads of this buffer must be done.
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
+    for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e1(t, k))] = p.mid(v[k]);
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e2(t, k))]);
     // Round 2: tile stages 3..5 pair element bits 4..2 = bits 2..0 of k.
 #pragma unroll
     for (int st = 3; st < 6; ++st) {
@@ -966,10 +973,10 @@ __device__ __forceinline__ void tiles_mac(const P& p, const KsArgs& a, int64_t g
     }
     // Each thread rewrites the words it just read: no barrier needed.
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
+    for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e2(t, k))] = p.mid(v[k]);
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e3(t, k))]);
     // Round 3: tile stages 6..7 pair element bits 1..0 = bits 1..0 of k.
 #pragma unroll
     for (int st = 6; st < 8; ++st) {
@@ -1125,11 +1132,11 @@ __device__ __forceinline__ void tiles_forward_store(const P& p, const KsArgs& a,
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const uint64_t h = (hi2[k >> 1] >> (16 * (k & 1))) & 0xffffu;
-      sh[tile_pad(mac_e3(t, k))] = __double_as_longlong(u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32)));
+      sh[mac_lds(mac_e3(t, k))] = __double_as_longlong(u64_to_f64(static_cast<uint64_t>(lo[k]) | (h << 32)));
     }
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e1(t, k))]);
   } else {
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = p.unmid(tile[mac_e1(t, k)]);
@@ -1145,10 +1152,10 @@ __device__ __forceinline__ void tiles_forward_store(const P& p, const KsArgs& a,
   }
   tile_sync();
 #pragma unroll
-  for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
+  for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e1(t, k))] = p.mid(v[k]);
   tile_sync();
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
+  for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e2(t, k))]);
 #pragma unroll
   for (int st = 3; st < 6; ++st) {
     const int half = 1 << (5 - st);
@@ -1159,10 +1166,10 @@ __device__ __forceinline__ void tiles_forward_store(const P& p, const KsArgs& a,
     }
   }
 #pragma unroll
-  for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
+  for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e2(t, k))] = p.mid(v[k]);
   tile_sync();
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
+  for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e3(t, k))]);
 #pragma unroll
   for (int st = 6; st < 8; ++st) {
     const int half = 1 << (7 - st);
@@ -1407,10 +1414,10 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
     }
     tile_sync();  // earlier readers of the exchange buffer are done
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e1(t, k))] = p.mid(v[k]);
+    for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e1(t, k))] = p.mid(v[k]);
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e2(t, k))]);
 #pragma unroll
     for (int st = 3; st < 6; ++st) {
       const int half = 1 << (5 - st);
@@ -1421,10 +1428,10 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
+    for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e2(t, k))] = p.mid(v[k]);
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e3(t, k))]);
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e3(t, k))]);
 #pragma unroll
     for (int st = 6; st < 8; ++st) {
       const int half = 1 << (7 - st);
@@ -1509,10 +1516,10 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e3(t, k))] = p.mid(v[k]);
+    for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e3(t, k))] = p.mid(v[k]);
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e2(t, k))]);
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e2(t, k))]);
 #pragma unroll
     for (int st = 3; st < 5; ++st) {
       const int half = 1 << (st - 2);
@@ -1524,10 +1531,10 @@ __device__ __forceinline__ void tiles_hmult(const P& p, const HmArgs& a, int64_t
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sh[tile_pad(mac_e2(t, k))] = p.mid(v[k]);
+    for (int k = 0; k < 8; ++k) sh[mac_lds(mac_e2(t, k))] = p.mid(v[k]);
     tile_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[tile_pad(mac_e1(t, k))]);
+    for (int k = 0; k < 8; ++k) v[k] = p.unmid(sh[mac_lds(mac_e1(t, k))]);
 #pragma unroll
     for (int st = 5; st < 8; ++st) {
       const int half = 1 << (st - 5);
