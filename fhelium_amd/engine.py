@@ -895,12 +895,7 @@ class CkksEngine:
         leading Q rows; scale and every other state axis are kept, and the
         result owns its storage (ckks_engine.py:1278-1320, 3209-3262)."""
 
-        if not isinstance(target_level, int):
-            raise TypeError("target_level must be an integer")
-        self.layout.check_level(target_level)
-        if target_level < ct.level:
-            raise ValueError(f"cannot mod-switch from level {ct.level} back to {target_level}")
-        drop = target_level - ct.level
+        drop = self._mod_switch_drop(ct, target_level)
         data = ct.data[..., drop:, :].contiguous()
         return Ciphertext(data, target_level, ct.scale, ct.polynomial_domain, ct.residue_representation,
                           ct.modulus_basis, ct.context_id)
@@ -910,10 +905,20 @@ class CkksEngine:
         becomes the view of the kept rows of its own allocation; the current
         level is a no-op."""
 
-        switched = self.mod_switch_to_level(ct, target_level)  # validates
-        ct.data = ct.data[..., target_level - ct.level:, :]
-        ct.level = switched.level
+        drop = self._mod_switch_drop(ct, target_level)
+        ct.data = ct.data[..., drop:, :]
+        ct.level = target_level
         return ct
+
+    def _mod_switch_drop(self, ct: Ciphertext, target_level: int) -> int:
+        """Leading Q rows a mod switch to ``target_level`` drops."""
+
+        if not isinstance(target_level, int) or isinstance(target_level, bool):
+            raise TypeError("target_level must be an integer")
+        self.layout.check_level(target_level)
+        if target_level < ct.level:
+            raise ValueError(f"cannot mod-switch from level {ct.level} back to {target_level}")
+        return target_level - ct.level
 
     @property
     def public_level_count(self) -> int:
